@@ -1,0 +1,663 @@
+"""Python side of the CPU oracle: ctypes bindings to adx_oracle.c plus the
+optimizer drivers restated from the reference in numpy.
+
+TEST INFRASTRUCTURE ONLY (see the package docstring).  Each driver cites the
+reference lines it follows (paths relative to the reference repository root).
+
+Parity status of each function is listed in ``PINS`` at the bottom: what
+known-answer data or mathematical property pins it, checked in
+``tests/test_oracle.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_BUILD = _HERE / "_build"
+
+_c_i64 = ctypes.c_int64
+_c_int = ctypes.c_int
+_c_dbl = ctypes.c_double
+_c_u64 = ctypes.c_uint64
+_ptr = ctypes.c_void_p
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle shared objects with the committed Makefile."""
+    if force or not (_BUILD / "liboracle.so").exists() or not (_BUILD / "liboracle_fast.so").exists():
+        subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = _BUILD / name
+    if not path.exists():
+        build()
+    return ctypes.CDLL(str(path))
+
+
+_LIB = None
+_LIB_FAST = None
+
+_SIGS = {
+    "oc_bed_to_packed": (None, [_ptr, _c_i64, _c_i64, _c_i64, _ptr, _c_i64]),
+    "oc_bed_to_unpacked": (None, [_ptr, _c_i64, _c_i64, _c_i64, _ptr]),
+    "oc_pack": (None, [_ptr, _c_i64, _c_i64, _ptr, _c_i64]),
+    "oc_unpack": (None, [_ptr, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _ptr]),
+    "oc_unpack_center": (None, [_ptr, _c_i64, _ptr, _c_i64, _c_i64, _c_i64, _c_i64, _ptr]),
+    "oc_mean_packed": (_c_dbl, [_ptr, _c_i64, _c_i64, _c_i64]),
+    "oc_flip_packed": (None, [_ptr, _c_i64, _c_i64, _c_i64]),
+    "oc_freq_cpu": (None, [_ptr, _c_i64, _c_i64, _ptr]),
+    "oc_freq_gpu": (None, [_ptr, _c_i64, _c_i64, _c_i64, _ptr]),
+    "oc_qbat": (None, [_ptr, _c_i64, _c_i64, _c_i64, _ptr]),
+    "oc_synth_packed": (None, [_ptr, _ptr, _c_i64, _c_i64, _c_int, _c_u64, _c_dbl, _ptr, _c_i64,
+                               _c_i64, _c_i64]),
+    "oc_em_stats_gpu": (None, [_ptr, _c_i64, _ptr, _ptr, _c_i64, _c_i64, _c_int, _ptr, _ptr, _ptr,
+                               _ptr]),
+    "oc_em_final_gpu": (None, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _c_i64, _c_i64, _c_int, _ptr,
+                               _ptr]),
+    "oc_adam_step_gpu": (None, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _c_i64, _c_i64,
+                                _c_int, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_dbl]),
+    "oc_loglik_gpu": (_c_dbl, [_ptr, _c_i64, _ptr, _ptr, _c_i64, _c_i64, _c_int]),
+    "oc_P_step_cpu": (None, [_ptr, _ptr, _ptr, _ptr, _ptr, _ptr, _c_int, _c_i64, _c_i64, _c_int]),
+    "oc_Q_step_cpu": (None, [_ptr, _ptr, _ptr, _ptr, _c_int, _c_i64]),
+    "oc_adamP_cpu": (None, [_ptr, _ptr, _ptr, _ptr, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_int, _c_i64,
+                            _c_int]),
+    "oc_adamQ_cpu": (None, [_ptr, _ptr, _ptr, _ptr, _c_dbl, _c_dbl, _c_dbl, _c_dbl, _c_int, _c_i64,
+                            _c_int]),
+    "oc_loglik_cpu": (_c_dbl, [_ptr, _ptr, _ptr, _c_i64, _c_i64, _c_int]),
+    "oc_mapQ": (None, [_ptr, _c_i64, _c_int]),
+    "oc_mapP": (None, [_ptr, _c_i64, _c_int]),
+    "oc_bvls_cpu": (None, [_ptr, _ptr, _ptr, _c_i64, _c_int, _c_dbl, _c_dbl]),
+    "oc_bvls_gpu": (None, [_ptr, _ptr, _ptr, _ptr, _c_i64, _c_int, _c_dbl, _c_dbl]),
+    "oc_qp_sweep": (_c_int, [_ptr, _ptr, _ptr, _c_int, _c_int]),
+    "oc_project_simplex": (None, [_ptr, _c_int, _c_dbl]),
+    "oc_sqp_q": (None, [_ptr, _ptr, _ptr, _ptr, _ptr, _c_i64, _c_int]),
+    "oc_sqp_p": (None, [_ptr, _ptr, _ptr, _ptr, _c_i64, _c_int]),
+    "oc_grad_hess_q": (None, [_ptr, _c_i64, _ptr, _ptr, _ptr, _ptr, _c_i64, _c_i64, _c_int]),
+    "oc_grad_hess_p": (None, [_ptr, _c_i64, _ptr, _ptr, _ptr, _ptr, _c_i64, _c_i64, _c_int]),
+    "oc_num_threads": (_c_int, []),
+}
+
+
+def _bind(lib: ctypes.CDLL) -> ctypes.CDLL:
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = _bind(_load("liboracle.so"))
+    return _LIB
+
+
+def lib_fast() -> ctypes.CDLL:
+    global _LIB_FAST
+    if _LIB_FAST is None:
+        _LIB_FAST = _bind(_load("liboracle_fast.so"))
+    return _LIB_FAST
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------
+# Genotype layout
+# --------------------------------------------------------------------------
+
+def bed_to_packed(bed: np.ndarray, M: int, N: int) -> np.ndarray:
+    """snp_reader.pyx:76-120 read_bed_packed; ``bed`` is M x ceil(N/4)."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    out = np.zeros(((M + 3) // 4, N), dtype=np.uint8)
+    lib().oc_bed_to_packed(_p(bed), M, bed.shape[1], N, _p(out), N)
+    return out
+
+
+def bed_to_unpacked(bed: np.ndarray, M: int, N: int) -> np.ndarray:
+    """snp_reader.pyx:35-73 read_bed."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    out = np.zeros((M, N), dtype=np.uint8)
+    lib().oc_bed_to_unpacked(_p(bed), M, bed.shape[1], N, _p(out))
+    return out
+
+
+def pack(G: np.ndarray) -> np.ndarray:
+    """snp_reader.pyx:123-156 pack_genotypes."""
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    M, N = G.shape
+    out = np.zeros(((M + 3) // 4, N), dtype=np.uint8)
+    lib().oc_pack(_p(G), M, N, _p(out), N)
+    return out
+
+
+def unpack(Gp: np.ndarray, M: int, m_start: int = 0, C: int | None = None) -> np.ndarray:
+    """pack2bit.cu:15-44: rows m_start .. m_start+C of the unpacked matrix."""
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    N = Gp.shape[1]
+    C = M - m_start if C is None else min(C, M - m_start)
+    out = np.zeros((C, N), dtype=np.uint8)
+    lib().oc_unpack(_p(Gp), N, M, N, m_start, C, _p(out))
+    return out
+
+
+def unpack_center(Gp: np.ndarray, f: np.ndarray, M: int, m_start: int, C: int) -> np.ndarray:
+    """pack2bit.cu:46-77."""
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    N = Gp.shape[1]
+    C = min(C, M - m_start)
+    out = np.zeros((C, N), dtype=np.float32)
+    lib().oc_unpack_center(_p(Gp), N, _p(f), M, N, m_start, C, _p(out))
+    return out
+
+
+def mean_packed(Gp: np.ndarray, M: int) -> float:
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    return lib().oc_mean_packed(_p(Gp), Gp.shape[1], M, Gp.shape[1])
+
+
+def flip_packed(Gp: np.ndarray, M: int) -> None:
+    """snp_reader.pyx:254-302, in place."""
+    assert Gp.flags["C_CONTIGUOUS"] and Gp.dtype == np.uint8
+    lib().oc_flip_packed(_p(Gp), Gp.shape[1], M, Gp.shape[1])
+
+
+def freq_cpu(G: np.ndarray) -> np.ndarray:
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    f = np.zeros(G.shape[0], dtype=np.float32)
+    with np.errstate(all="ignore"):
+        lib().oc_freq_cpu(_p(G), G.shape[0], G.shape[1], _p(f))
+    return f
+
+
+def freq_gpu(Gp: np.ndarray, M: int) -> np.ndarray:
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    f = np.zeros(M, dtype=np.float32)
+    lib().oc_freq_gpu(_p(Gp), Gp.shape[1], M, Gp.shape[1], _p(f))
+    return f
+
+
+def qbat(Gp: np.ndarray, M: int) -> np.ndarray:
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    out = np.zeros(Gp.shape[1], dtype=np.float64)
+    lib().oc_qbat(_p(Gp), Gp.shape[1], M, Gp.shape[1], _p(out))
+    return out
+
+
+def synth_packed(P: np.ndarray, Q: np.ndarray, seed: int, miss_rate: float,
+                 row0: int = 0, rows: int | None = None) -> np.ndarray:
+    """CPU twin of the device genotype generator (adx_synth_packed)."""
+    P = _f64(P)
+    Q = _f64(Q)
+    M, K = P.shape
+    N = Q.shape[0]
+    Mb = (M + 3) // 4
+    rows = Mb - row0 if rows is None else min(rows, Mb - row0)
+    out = np.zeros((rows, N), dtype=np.uint8)
+    lib().oc_synth_packed(_p(P), _p(Q), M, N, K, seed & 0xFFFFFFFFFFFFFFFF, miss_rate,
+                          _p(out), N, row0, rows)
+    return out
+
+
+# --------------------------------------------------------------------------
+# Adam-EM, GPU semantics (em_adam_gpu.py)
+# --------------------------------------------------------------------------
+
+def em_stats_gpu(Gp, P, Q, M):
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    P = _f64(P)
+    Q = _f64(Q)
+    K = P.shape[1]
+    N = Q.shape[0]
+    A = np.zeros((M, K))
+    B = np.zeros((M, K))
+    T = np.zeros((N, K))
+    qb = np.zeros(N)
+    lib().oc_em_stats_gpu(_p(Gp), Gp.shape[1], _p(P), _p(Q), M, N, K, _p(A), _p(B), _p(T), _p(qb))
+    return A, B, T, qb
+
+
+def em_final_gpu(P, Q, A, B, T, qb):
+    P, Q, A, B, T, qb = map(_f64, (P, Q, A, B, T, qb))
+    M, K = P.shape
+    N = Q.shape[0]
+    PE = np.zeros_like(P)
+    QE = np.zeros_like(Q)
+    lib().oc_em_final_gpu(_p(P), _p(Q), _p(A), _p(B), _p(T), _p(qb), M, N, K, _p(PE), _p(QE))
+    return PE, QE
+
+
+def loglik_gpu(Gp, P, Q, M) -> float:
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    P = _f64(P)
+    Q = _f64(Q)
+    return lib().oc_loglik_gpu(_p(Gp), Gp.shape[1], _p(P), _p(Q), M, Q.shape[0], P.shape[1])
+
+
+class AdamEMGPU:
+    """em_adam_gpu.py:121-224 EMAdamOptimizer restated on numpy buffers."""
+
+    def __init__(self, M, N, K, lr, beta1, beta2, eps):
+        self.lr, self.b1, self.b2, self.eps = lr, beta1, beta2, eps
+        self.t = 0.0
+        self.mP = np.zeros((M, K))
+        self.vP = np.zeros((M, K))
+        self.mQ = np.zeros((N, K))
+        self.vQ = np.zeros((N, K))
+
+    def run_em_step(self, Gp, P, Q, M):
+        A, B, T, qb = em_stats_gpu(Gp, P, Q, M)
+        return em_final_gpu(P, Q, A, B, T, qb)
+
+    def step(self, P, Q, PE, QE):
+        self.t += 1.0
+        M, K = P.shape
+        lib().oc_adam_step_gpu(_p(P), _p(Q), _p(_f64(PE)), _p(_f64(QE)), _p(self.mP), _p(self.vP),
+                               _p(self.mQ), _p(self.vQ), M, Q.shape[0], K, self.t, self.lr,
+                               self.b1, self.b2, self.eps)
+
+
+def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_iter=10000,
+               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None):
+    """em_adam_gpu.py:281-356 _optimize_parameters_gpu_once.
+
+    Returns (P_best, Q_best).  If ``history`` is a list, (iteration, logL)
+    pairs of every check are appended to it.  The third priming call of the
+    reference (:322) only refills the EM target buffers, so it is omitted.
+    """
+    P = _f64(P).copy()
+    Q = _f64(Q).copy()
+    N, K = Q.shape
+    opt = AdamEMGPU(M, N, K, lr, beta1, beta2, reg_adam)
+    PE, QE = opt.run_em_step(Gp, P, Q, M)
+    opt.step(P, Q, PE, QE)
+    L_best = loglik_gpu(Gp, P, Q, M)
+    if history is not None:
+        history.append((0, L_best))
+    P_best, Q_best = P.copy(), Q.copy()
+    wait = 0
+    for it in range(max_iter):
+        PE, QE = opt.run_em_step(Gp, P, Q, M)
+        opt.step(P, Q, PE, QE)
+        if (it + 1) % check == 0:
+            L = loglik_gpu(Gp, P, Q, M)
+            if history is not None:
+                history.append((it + 1, L))
+            if L > L_best + tol:
+                L_best = L
+                P_best[:] = P
+                Q_best[:] = Q
+                wait = 0
+            else:
+                wait += 1
+                if wait >= patience:
+                    opt.lr = max(opt.lr * lr_decay, min_lr)
+                    if opt.lr <= min_lr:
+                        break
+                    wait = 0
+    return P_best, Q_best
+
+
+# --------------------------------------------------------------------------
+# Adam-EM, CPU semantics (em_adam.py, em.pyx, tools.pyx)
+# --------------------------------------------------------------------------
+
+def loglik_cpu(G, P, Q) -> float:
+    """tools.pyx:59-85 on unpacked G."""
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    P = _f64(P)
+    Q = _f64(Q)
+    with np.errstate(all="ignore"):
+        return lib().oc_loglik_cpu(_p(G), _p(P), _p(Q), G.shape[0], G.shape[1], P.shape[1])
+
+
+def em_step_cpu(G, P0, Q0, T, P1, Q1, qb, nthreads=1, fast=False):
+    """em_adam.py:56-80 emStep core: P_step then Q_step (targets in P1/Q1)."""
+    L = lib_fast() if fast else lib()
+    M, K = P0.shape
+    N = Q0.shape[0]
+    L.oc_P_step_cpu(_p(G), _p(P0), _p(P1), _p(Q0), _p(T), _p(qb), K, M, N, nthreads)
+    L.oc_Q_step_cpu(_p(Q0), _p(Q1), _p(T), _p(qb), K, N)
+
+
+def adamem_cpu(G, P, Q, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_iter=10000,
+               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None):
+    """em_adam.py:82-179 optimize_parameters (CPU Adam-EM)."""
+    G = np.ascontiguousarray(G, dtype=np.uint8)
+    P = _f64(P).copy()
+    Q = _f64(Q).copy()
+    M, K = P.shape
+    N = Q.shape[0]
+    mP, vP, mQ, vQ = (np.zeros_like(P), np.zeros_like(P), np.zeros_like(Q), np.zeros_like(Q))
+    P1, Q1, T, qb = np.zeros_like(P), np.zeros_like(Q), np.zeros_like(Q), np.zeros(N)
+    t = 0
+
+    def em_step():
+        em_step_cpu(G, P, Q, T, P1, Q1, qb)
+        P[:] = P1
+        Q[:] = Q1
+
+    def adam_step():
+        nonlocal t
+        em_step_cpu(G, P, Q, T, P1, Q1, qb)
+        t += 1
+        lib().oc_adamP_cpu(_p(P), _p(P1), _p(mP), _p(vP), lr_cur[0], beta1, beta2, reg_adam, t, M, K)
+        lib().oc_adamQ_cpu(_p(Q), _p(Q1), _p(mQ), _p(vQ), lr_cur[0], beta1, beta2, reg_adam, t, N, K)
+
+    lr_cur = [lr]
+    em_step()
+    adam_step()
+    em_step()
+    L_best = loglik_cpu(G, P, Q)
+    if history is not None:
+        history.append((0, L_best))
+    P_best, Q_best = P.copy(), Q.copy()
+    wait = 0
+    for it in range(max_iter):
+        adam_step()
+        if (it + 1) % check == 0:
+            L = loglik_cpu(G, P, Q)
+            if history is not None:
+                history.append((it + 1, L))
+            if L > L_best + tol:
+                L_best = L
+                P_best[:] = P
+                Q_best[:] = Q
+                wait = 0
+            else:
+                wait += 1
+                if wait >= patience:
+                    lr_cur[0] = max(lr_cur[0] * lr_decay, min_lr)
+                    if lr_cur[0] <= min_lr:
+                        break
+                    wait = 0
+    return P_best, Q_best
+
+
+# --------------------------------------------------------------------------
+# Randomized SVD and ALS initialisation (svd.py, svd_gpu.py, als.py, als_gpu.py)
+# --------------------------------------------------------------------------
+
+def eig_svd(X: np.ndarray):
+    """svd.py:13-28 eigSVD: SVD of X (m x n) via eigh of X^T X, descending."""
+    D, V = np.linalg.eigh(X.T @ X)
+    S = np.sqrt(D)
+    U = X @ (V * (1.0 / S))
+    return (np.ascontiguousarray(U[:, ::-1]), np.ascontiguousarray(S[::-1]),
+            np.ascontiguousarray(V[:, ::-1]))
+
+
+def rsvd(X_rows, M: int, N: int, k: int, seed: int, power: int, tol: float, chunk: int,
+         proj_basis: np.ndarray | None = None):
+    """svd.py:30-140 _RSVD_once on the centred genotype matrix.
+
+    ``X_rows(start, n)`` returns the centred FP32 rows [start, start+n) (the
+    decompress_block of tools.pyx:292-324).  ``proj_basis`` (M x L float32)
+    replaces the seeded standard-normal draw when given.
+    """
+    rng = np.random.default_rng(seed)
+    n_batches = int(np.ceil(M / chunk))
+    alpha = 0.0
+    L = max(k + 10, 20)
+    accum = np.zeros((N, L), dtype=np.float32)
+    if proj_basis is None:
+        proj_basis = rng.standard_normal(size=(M, L), dtype=np.float32)
+    else:
+        proj_basis = np.array(proj_basis, dtype=np.float32, copy=True)
+    for w in range(n_batches):
+        s = w * chunk
+        n = min(chunk, M - s)
+        accum += X_rows(s, n).T @ proj_basis[s:s + n]
+    orth, _, _ = eig_svd(accum)
+    accum.fill(0.0)
+    sv = np.zeros(L, dtype=np.float32)
+    for i in range(power):
+        for w in range(n_batches):
+            s = w * chunk
+            n = min(chunk, M - s)
+            Xc = X_rows(s, n)
+            proj_basis[s:s + n] = Xc @ orth
+            accum += Xc.T @ proj_basis[s:s + n]
+        accum -= alpha * orth
+        orth, s_vals, _ = eig_svd(accum)
+        accum.fill(0.0)
+        if i > 0:
+            s_cur = s_vals + alpha
+            rel = np.abs(s_cur - sv[:len(s_cur)]) / np.maximum(s_cur, 1e-12)
+            if np.max(rel[:k]) < tol:
+                break
+            sv[:len(s_cur)] = s_cur
+        else:
+            sv[:len(s_vals)] = s_vals + alpha
+        if s_vals[-1] > alpha:
+            alpha = 0.5 * (s_vals[-1] + alpha)
+    for w in range(n_batches):
+        s = w * chunk
+        n = min(chunk, M - s)
+        proj_basis[s:s + n] = X_rows(s, n) @ orth
+    u, s_all, v = eig_svd(proj_basis)
+    return (np.ascontiguousarray(u[:, :k]), np.ascontiguousarray(s_all[:k]),
+            np.ascontiguousarray(orth @ v[:, :k]))
+
+
+def bvls_cpu(A, B, lo=0.0, hi=1.0):
+    """bvls.pyx:268-312 batch_bvls_bpp."""
+    A = _f64(A)
+    B = _f64(B)
+    X = np.zeros_like(B)
+    lib().oc_bvls_cpu(_p(A), _p(B), _p(X), B.shape[0], B.shape[1], lo, hi)
+    return X
+
+
+def bvls_gpu(A, B, lo=0.0, hi=1.0):
+    """bvls_kernel.cu:188-218 batch_bvls_bpp_cuda: X0 = B inv(A)^T, then BPP."""
+    A = _f64(A)
+    B = _f64(B)
+    X0 = _f64(B @ np.linalg.inv(A).T)
+    X = np.zeros_like(B)
+    lib().oc_bvls_gpu(_p(A), _p(B), _p(X0), _p(X), B.shape[0], B.shape[1], lo, hi)
+    return X
+
+
+def map_p(P):
+    P = _f64(P)
+    lib().oc_mapP(_p(P), P.shape[0], P.shape[1])
+    return P
+
+
+def map_q(Q):
+    Q = _f64(Q)
+    lib().oc_mapQ(_p(Q), Q.shape[0], Q.shape[1])
+    return Q
+
+
+def als(U, S, V, f, seed, M, N, K, max_iter, tol, P_init=None, gpu_semantics=False):
+    """als.py:10-88 ALS (CPU), or als_gpu.py:45-127 when ``gpu_semantics``.
+
+    ``P_init`` (M x K) replaces the seeded uniform draw (the GPU path draws
+    it from torch's device generator, als_gpu.py:69-86, which no CPU code can
+    reproduce)."""
+    Z = _f64(U * S)
+    V = _f64(V)
+    f64 = np.asarray(f, dtype=np.float64)
+    if P_init is None:
+        P = np.random.default_rng(seed).random(size=(M, K), dtype=np.float64)
+    else:
+        P = _f64(P_init).copy()
+    P = map_p(P)
+    solver = bvls_gpu if gpu_semantics else bvls_cpu
+    I_p = P @ np.linalg.pinv(P.T @ P)
+    Q = 0.5 * (V @ (Z.T @ I_p)) + (f64 @ I_p)
+    Q = map_q(Q)
+    Q0 = Q.copy()
+    for _ in range(max_iter):
+        A_q = Q.T @ Q
+        I_q = Q @ np.linalg.pinv(A_q)
+        P_free = 0.5 * (Z @ (V.T @ I_q)) + np.outer(f64, I_q.sum(axis=0))
+        P = map_p(solver(A_q, P_free @ A_q))
+        A_p = P.T @ P
+        I_p = P @ np.linalg.pinv(A_p)
+        Q_free = 0.5 * (V @ (Z.T @ I_p)) + (f64 @ I_p)
+        Q = map_q(solver(A_p, Q_free @ A_p))
+        rmse = np.sqrt(np.mean((Q - Q0) ** 2))
+        if rmse < tol:
+            break
+        Q0[:] = Q
+    return P, Q
+
+
+# --------------------------------------------------------------------------
+# SQP + ZAL quasi-Newton (sqp.pyx, br_qn.pyx, br_qn.py, br_qn_gpu.py)
+# --------------------------------------------------------------------------
+
+def v_kk(K: int) -> np.ndarray:
+    """br_qn.py:98-99: right singular vectors of ones(1, K), transposed."""
+    _, _, vt = np.linalg.svd(np.ones((1, K)), full_matrices=True)
+    return np.ascontiguousarray(vt.T, dtype=np.float64)
+
+
+def grad_hess_q(Gp, Q, P, M):
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    Q = _f64(Q)
+    P = _f64(P)
+    N, K = Q.shape
+    H = np.zeros((N, K, K))
+    z = np.zeros((N, K))
+    lib().oc_grad_hess_q(_p(Gp), Gp.shape[1], _p(Q), _p(P), _p(H), _p(z), M, N, K)
+    return H, z
+
+
+def grad_hess_p(Gp, Q, P, M):
+    Gp = np.ascontiguousarray(Gp, dtype=np.uint8)
+    Q = _f64(Q)
+    P = _f64(P)
+    N, K = Q.shape
+    H = np.zeros((M, K, K))
+    z = np.zeros((M, K))
+    lib().oc_grad_hess_p(_p(Gp), Gp.shape[1], _p(Q), _p(P), _p(H), _p(z), M, N, K)
+    return H, z
+
+
+def sqp_q(H, z, Q, vkk):
+    H, z, Q, vkk = map(_f64, (H, z, Q, vkk))
+    out = np.zeros_like(Q)
+    lib().oc_sqp_q(_p(H), _p(z), _p(Q), _p(out), _p(vkk), Q.shape[0], Q.shape[1])
+    return out
+
+
+def sqp_p(H, z, P):
+    H, z, P = map(_f64, (H, z, P))
+    out = np.zeros_like(P)
+    lib().oc_sqp_p(_p(H), _p(z), _p(P), _p(out), P.shape[0], P.shape[1])
+    return out
+
+
+def qp_sweep(tableau: np.ndarray, par: np.ndarray, p: int, c: int):
+    tab = _f64(tableau).copy()
+    par = _f64(par)
+    delta = np.zeros(p)
+    it = lib().oc_qp_sweep(_p(delta), _p(tab), _p(par), p, c)
+    return delta, it
+
+
+def project_simplex(b: np.ndarray, pc: float = 1e-5) -> np.ndarray:
+    b = _f64(b).copy()
+    lib().oc_project_simplex(_p(b), b.shape[0], pc)
+    return b
+
+
+def zal_extrapolate(U: np.ndarray, V: np.ndarray, x: np.ndarray, x_next: np.ndarray) -> np.ndarray:
+    """br_qn.pyx:22-89 qn_extrapolate_ZAL with columns U, V (dim x q)."""
+    lhs = U.T @ (U - V)
+    rhs = U.T @ (x - x_next)
+    alpha = np.linalg.solve(lhs, rhs)
+    return x_next - V @ alpha
+
+
+def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None, history=None):
+    """br_qn.py:73-205 optimize_original (CPU semantics, packed G)."""
+    P = _f64(P).copy()
+    Q = _f64(Q).copy()
+    N, K = Q.shape
+    vkk = v_kk(K) if vkk is None else _f64(vkk)
+    dim = (M + N) * K
+    U = np.zeros((dim, Q_hist))
+    V = np.zeros((dim, Q_hist))
+    ll_prev = -np.inf
+    ll_best = -np.inf
+    wait = 0
+    P_best, Q_best = P.copy(), Q.copy()
+    for it in range(1, max_iter + 1):
+        H, z = grad_hess_q(Gp, Q, P, M)
+        Qn = sqp_q(H, z, Q, vkk)
+        H, z = grad_hess_p(Gp, Qn, P, M)
+        Pn = sqp_p(H, z, P)
+        H, z = grad_hess_q(Gp, Qn, Pn, M)
+        Qn2 = sqp_q(H, z, Qn, vkk)
+        H, z = grad_hess_p(Gp, Qn2, Pn, M)
+        Pn2 = sqp_p(H, z, Pn)
+        x = np.concatenate([P.ravel(), Q.ravel()])
+        x1 = np.concatenate([Pn.ravel(), Qn.ravel()])
+        x2 = np.concatenate([Pn2.ravel(), Qn2.ravel()])
+        col = (it - 1) % Q_hist
+        U[:, col] = x1 - x
+        V[:, col] = x2 - x1
+        nc = min(it, Q_hist)
+        xq = zal_extrapolate(U[:, :nc], V[:, :nc], x, x1)
+        Pq = np.clip(xq[:M * K].reshape(M, K), 1e-5, 1 - 1e-5)
+        Qq = xq[M * K:].reshape(N, K).copy()
+        for i in range(N):
+            Qq[i] = project_simplex(Qq[i])
+        llq = loglik_gpu(Gp, Pq, Qq, M)
+        if llq > ll_prev:
+            P, Q, ll_new = Pq, Qq, llq
+        else:
+            P, Q = Pn2, Qn2
+            ll_new = loglik_gpu(Gp, P, Q, M)
+        best_before = ll_best
+        if ll_new > ll_best:
+            ll_best = ll_new
+            P_best, Q_best = P.copy(), Q.copy()
+        if best_before != -np.inf and ll_new <= best_before + tol:
+            wait += 1
+        else:
+            wait = 0
+        if history is not None:
+            history.append((it, ll_new))
+        if abs(ll_new - ll_prev) < tol or wait >= patience:
+            break
+        ll_prev = ll_new
+    return P_best, Q_best
+
+
+PINS = {
+    "bed_to_packed / bed_to_unpacked / unpack / pack": "reader fixtures demo_data.G(.packed).expected, exact",
+    "flip_packed / mean_packed": "closed form on random tiles (0<->2, padding zeroed) and demo fixture",
+    "unpack_center / freq_gpu / freq_cpu / qbat": "numpy closed form on random data",
+    "synth_packed": "binomial moments (mean 2h, missing rate) on a large block",
+    "em_stats_gpu / em_final_gpu / loglik_gpu": "pure-Python brute force on tiny inputs; EM ascent; "
+                                                  "fixture trajectory outputs/gpu/adamem",
+    "adamem_gpu": "outputs/gpu/adamem/demo_run.adamem.* from outputs/gpu/adamem/demo_run.als.*",
+    "adamem_cpu / em_step_cpu / loglik_cpu": "outputs/cpu/adamem/demo_run.adamem.* from "
+                                             "outputs/cpu/brqn/demo_run.als.*",
+    "rsvd / eig_svd": "dense numpy SVD on small matrices; outputs/*/svd.* fixtures",
+    "als / bvls_cpu": "outputs/cpu/brqn/demo_run.als.* from the svd fixtures; KKT; scipy lsq_linear",
+    "bvls_gpu": "KKT + scipy lsq_linear",
+    "qp_sweep / sqp_q / sqp_p / project_simplex": "KKT, brute-force active sets, sort-threshold projection",
+    "grad_hess_q / grad_hess_p": "finite differences of loglik_gpu",
+    "brqn / zal_extrapolate": "outputs/cpu/brqn/demo_run.brqn.* from the als fixtures",
+}
