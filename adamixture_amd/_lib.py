@@ -1,0 +1,86 @@
+"""ctypes binding of libadx.so (the gfx950 kernels, C ABI in include/adx.h).
+
+The library is built in-tree by ``adamixture_amd._build.build_library`` and
+loaded after ``import torch`` so that it resolves ``libamdhip64.so.7`` to the
+HIP runtime torch already loaded (one runtime per process).  There is no
+fallback: if the library is missing, every GPU entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede loading libadx: shared HIP runtime)
+
+LIB_PATH = Path(__file__).resolve().parent / "libadx.so"
+
+ADX_F32 = 0
+ADX_F64 = 1
+ADX_BF16 = 2
+ADX_EM_STATS = 1
+ADX_EM_LOGLIK = 2
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_u64 = ctypes.c_uint64
+_p = ctypes.c_void_p
+
+# name -> (restype, argtypes); mirrors include/adx.h
+SIGNATURES = {
+    "adx_last_error": (ctypes.c_char_p, []),
+    "adx_unpack_u8": (_int, [_p, _i64, _i64, _i64, _i64, _i64, _i64, _p, _p]),
+    "adx_unpack_center_f32": (_int, [_p, _i64, _p, _i64, _i64, _i64, _i64, _i64, _p, _p]),
+    "adx_freq_f32": (_int, [_p, _i64, _i64, _i64, _p, _p]),
+    "adx_qbat": (_int, [_p, _i64, _i64, _i64, _p, _p]),
+    "adx_repack_padded": (_int, [_p, _i64, _i64, _i64, _p, _i64, _p]),
+    "adx_synth_packed": (_int, [_p, _p, _i64, _i64, _int, _u64, _dbl, _p, _i64, _p]),
+    "adx_em_workspace_bytes": (_i64, [_int, _i64, _i64, _int]),
+    "adx_em_pass": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _i64, _p, _i64, _p, _p, _p, _int, _p,
+                           _p, _i64, _p]),
+    "adx_em_finalize": (_int, [_int, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _p, _p]),
+    "adx_adam_step": (_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _dbl, _dbl, _dbl, _dbl,
+                             _dbl, _p, _p, _i64, _p]),
+    "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),
+}
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load and bind libadx.so; raises if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"libadx.so not found at {LIB_PATH}; build it with "
+                    "`python -c 'import adamixture_amd._build as b; b.build_library()'`")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+        return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = load().adx_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({status}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def stream_handle(device: torch.device | None = None) -> int:
+    """HIP stream of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()

@@ -1,0 +1,59 @@
+// Shared helpers for the gfx950 kernels of libadx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+namespace adx {
+
+// Last error message of the calling host thread (returned by adx_last_error).
+void set_error(const std::string& msg);
+
+// Translate a HIP status into the library status.  Out-of-memory keeps the
+// phrase "out of memory" in the message so Python callers can classify it.
+int fail(hipError_t err, const char* where);
+int fail_msg(int code, const std::string& msg);
+
+#define ADX_CHECK(expr)                                   \
+    do {                                                  \
+        hipError_t _e = (expr);                           \
+        if (_e != hipSuccess) return adx::fail(_e, #expr); \
+    } while (0)
+
+#define ADX_LAUNCH_CHECK(where)                            \
+    do {                                                   \
+        hipError_t _e = hipGetLastError();                 \
+        if (_e != hipSuccess) return adx::fail(_e, where); \
+    } while (0)
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// One 16x16x4 MFMA step; identical lane layouts for FP32 and FP64:
+//   A (16x4):  lane l supplies A[l % 16][l / 16]
+//   B (4x16):  lane l supplies B[l / 16][l % 16]
+//   D (16x16): lane l holds D[4 * (l / 16) + j][l % 16], j = 0..3
+template <typename T>
+struct Mma16;
+
+template <>
+struct Mma16<float> {
+    using acc = floatx4;
+    __device__ static inline acc run(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mma16<double> {
+    using acc = doublex4;
+    __device__ static inline acc run(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+}  // namespace adx

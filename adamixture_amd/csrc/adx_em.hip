@@ -1,0 +1,483 @@
+// Fused genotype-decode + EM-statistics pass, EM finalize and Adam step.
+//
+// Reference: adamixture/model/em_adam_gpu.py.  The reference materialises
+// six C x N FP64 temporaries per SNP chunk and runs four skinny GEMMs
+// (em_batch_math, :46-78).  Here one launch streams the 2-bit genotypes once
+// and keeps every intermediate in registers:
+//
+//   tile (16 samples x 16 SNPs), formulation "H^T" (rows = samples):
+//     H^T  = Q_tile . P_tile^T                     MFMA, k = K (padded to 4)
+//     a, b = g/H, (2-g)/(1-H) on observed codes    VALU, in accumulator layout
+//     A^T += Q_tile^T . a^T,  B^T += Q_tile^T . b^T  MFMA, k = samples; the
+//            accumulator layout of H^T is exactly the B-operand layout here
+//     T   += c^T . P_tile (c = a - b)              MFMA, k = SNPs; c^T goes
+//            through a 1 KB per-wave LDS transpose
+//     T   += colsum_SNP(b)                         VALU + 16-lane shuffles
+//
+// Work split: a workgroup (4 waves) owns 64*CW consecutive SNPs and one
+// sample strip; each wave owns 16*CW SNPs = 4*CW byte rows.  Within a 64-
+// sample sub-tile, lane (g, c) loads 16 contiguous bytes (samples
+// s0+16g .. s0+16g+15) of byte row 16t+c, so every genotype byte is read
+// once, coalesced, and supplies the four SNP column blocks cb = 4t..4t+3.
+// A and B stay in MFMA accumulators for the whole strip; T is reduced per
+// sub-tile across the 4 waves in LDS and added to global memory with one
+// atomic per (sample, k) per workgroup.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "adx_common.h"
+#include "../../include/adx.h"
+
+namespace adx {
+
+template <typename T>
+__device__ __forceinline__ void em_terms(T h, uint32_t code, T& a, T& b);
+
+// FP32: one reciprocal of h(1-h) gives both 1/h and 1/(1-h).
+template <>
+__device__ __forceinline__ void em_terms<float>(float h, uint32_t code, float& a, float& b) {
+    const float hc = fminf(fmaxf(h, 1e-5f), 1.0f - 1e-5f);
+    const float u = 1.0f - hc;
+    const float r = __builtin_amdgcn_rcpf(hc * u);
+    const float ca = code == 3u ? 0.0f : (float)code;
+    const float cb = code == 3u ? 0.0f : 2.0f - (float)code;
+    a = ca * (u * r);
+    b = cb * (hc * r);
+}
+
+// FP64: the reference's divisions (em_adam_gpu.py:62-67).
+template <>
+__device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double& a, double& b) {
+    const double hc = fmin(fmax(h, 1e-5), 1.0 - 1e-5);
+    const double g = (double)code;
+    a = code == 3u ? 0.0 : g / hc;
+    b = code == 3u ? 0.0 : (2.0 - g) / (1.0 - hc);
+}
+
+// em_adam_gpu.py:238-246: H clamped to [1e-10, 1-1e-10], FP64 logs.
+__device__ __forceinline__ double ll_term(double h, uint32_t code) {
+    if (code == 3u) return 0.0;
+    const double hc = fmin(fmax(h, 1e-10), 1.0 - 1e-10);
+    const double g = (double)code;
+    return g * log(hc) + (2.0 - g) * log1p(-hc);
+}
+
+template <typename T, int KS, int CW, bool STATS, bool LL>
+__global__ __launch_bounds__(256) void em_pass_kernel(
+    const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
+    int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
+    T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part) {
+    using MM = Mma16<T>;
+    using acc_t = typename MM::acc;
+    static_assert(CW % 4 == 0, "CW must be a multiple of 4");
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
+    const int64_t snp_w = ((int64_t)blockIdx.x * 4 + wave) * (16 * CW);
+    const int64_t row_w = snp_w >> 2;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
+    const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
+
+    __shared__ T s_tile[4][256];
+    __shared__ T s_tsum[64 * 16];
+    __shared__ double s_ll[4];
+
+    if (STATS)
+        for (int e = threadIdx.x; e < 64 * 16; e += 256) s_tsum[e] = T(0);
+
+    // P operands, constant for the whole strip.
+    T pH[CW][KS];
+    T pT[CW][4];
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        const int t = cb >> 2, f = cb & 3;
+        const int64_t snp = snp_w + 64 * t + 4 * cl + f;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int kp = 4 * ks + g;
+            pH[cb][ks] = (snp < M && kp < K) ? P[snp * ldp + kp] : T(0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int64_t snp2 = snp_w + 64 * t + 4 * (4 * kk + g) + f;
+            pT[cb][kk] = (STATS && snp2 < M && cl < K) ? P[snp2 * ldp + cl] : T(0);
+        }
+    }
+
+    acc_t accA[CW], accB[CW];
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        accA[cb] = acc_t{0, 0, 0, 0};
+        accB[cb] = acc_t{0, 0, 0, 0};
+    }
+    double ll = 0.0;
+    __syncthreads();
+
+    for (int64_t s0 = n0; s0 < n1; s0 += 64) {
+        // Genotype bytes: 16 samples of byte row 16t+cl per t.
+        uint32_t gw[CW / 4][4];
+#pragma unroll
+        for (int t = 0; t < CW / 4; ++t) {
+            const int64_t row = row_w + 16 * t + cl;
+            if (row < Mb) {
+                const uint4 v = *reinterpret_cast<const uint4*>(G + row * ldg + s0 + 16 * g);
+                gw[t][0] = v.x;
+                gw[t][1] = v.y;
+                gw[t][2] = v.z;
+                gw[t][3] = v.w;
+                const int64_t nf = M - 4 * row;
+                if (nf < 4) {  // SNP fields beyond M read as missing
+                    const uint32_t pm = ((0xFFu << (2 * nf)) & 0xFFu) * 0x01010101u;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) gw[t][r] |= pm;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) gw[t][r] = 0xFFFFFFFFu;
+            }
+        }
+
+        // Row blocks of 16 samples: sample offset of D-row 4g'+j in row block
+        // rw is 16g'+4rw+j, so lane (g, cl) reads bytes 4rw..4rw+3 of its 16.
+#pragma unroll
+        for (int rw = 0; rw < 4; ++rw) {
+            T qH[KS];
+            T qT[4];
+            const int64_t sH = s0 + 16 * (cl >> 2) + 4 * rw + (cl & 3);
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int kp = 4 * ks + g;
+                qH[ks] = (sH < N && kp < K) ? Q[sH * ldq + kp] : T(0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int64_t sT = s0 + 16 * g + 4 * rw + j;
+                qT[j] = (STATS && sT < N && cl < K) ? Q[sT * ldq + cl] : T(0);
+            }
+            acc_t accT = acc_t{0, 0, 0, 0};
+            T sb[4] = {T(0), T(0), T(0), T(0)};
+
+#pragma unroll
+            for (int cb = 0; cb < CW; ++cb) {
+                const int t = cb >> 2, f = cb & 3;
+                acc_t h = acc_t{0, 0, 0, 0};
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) h = MM::run(qH[ks], pH[cb][ks], h);
+                const uint32_t w = gw[t][rw] >> (2 * f);
+                T a[4], b[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t code = (w >> (8 * j)) & 3u;
+                    if (STATS) em_terms<T>(h[j], code, a[j], b[j]);
+                    if (LL) ll += ll_term((double)h[j], code);
+                }
+                if (STATS) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        accA[cb] = MM::run(qT[j], a[j], accA[cb]);
+                        accB[cb] = MM::run(qT[j], b[j], accB[cb]);
+                        s_tile[wave][cl * 16 + 4 * g + j] = a[j] - b[j];
+                        sb[j] += b[j];
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk)
+                        accT = MM::run(s_tile[wave][(4 * kk + g) * 16 + cl], pT[cb][kk], accT);
+                }
+            }
+
+            if (STATS) {
+                // colsum(b) over the wave's SNPs: reduce across the 16 lanes of a group.
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    T v = sb[j];
+                    v += __shfl_xor(v, 1);
+                    v += __shfl_xor(v, 2);
+                    v += __shfl_xor(v, 4);
+                    v += __shfl_xor(v, 8);
+                    if (cl < K) atomicAdd(&s_tsum[(16 * g + 4 * rw + j) * 16 + cl], accT[j] + v);
+                }
+            }
+        }
+
+        if (STATS) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+                const int kp = e & 15;
+                const int64_t smp = s0 + (e >> 4);
+                if (kp < K && smp < N) atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+                s_tsum[e] = T(0);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (STATS) {
+#pragma unroll
+        for (int cb = 0; cb < CW; ++cb) {
+            const int t = cb >> 2, f = cb & 3;
+            const int64_t snp = snp_w + 64 * t + 4 * cl + f;
+            if (snp < M) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int kp = 4 * g + jj;
+                    if (kp < K) {
+                        atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
+                        atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
+                    }
+                }
+            }
+        }
+    }
+
+    if (LL) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off);
+        if (lane == 0) s_ll[wave] = ll;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            ll_part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = (s_ll[0] + s_ll[1]) + (s_ll[2] + s_ll[3]);
+    }
+}
+
+// Deterministic sum of n FP64 partials (fixed order) into out[0].
+__global__ __launch_bounds__(256) void reduce_f64_kernel(const double* __restrict__ x, int64_t n,
+                                                         double* __restrict__ out) {
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) v += x[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// em_adam_gpu.py:101-107 (P side), elementwise; zeroes A and B.
+template <typename TA>
+__global__ void finalize_p_kernel(const double* __restrict__ P, TA* __restrict__ A, TA* __restrict__ B,
+                                  int64_t n, double* __restrict__ P_EM) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const double a = (double)A[x], b = (double)B[x], p = P[x];
+    double d = (a - b) * p + b;
+    d = fmax(d, 1e-5);
+    P_EM[x] = ((1.0 / d) * p) * a;
+    A[x] = TA(0);
+    B[x] = TA(0);
+}
+
+// em_adam_gpu.py:109-115 (Q side), one thread per sample; zeroes T.
+template <typename TA>
+__global__ void finalize_q_kernel(const double* __restrict__ Q, TA* __restrict__ T,
+                                  const double* __restrict__ qbat, int64_t N, int K,
+                                  double* __restrict__ Q_EM) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double sc = 1.0 / fmax(qbat[i], 1e-5);
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const double v = (Q[i * K + k] * (double)T[i * K + k]) * sc;
+        Q_EM[i * K + k] = v;
+        s += v;
+        T[i * K + k] = TA(0);
+    }
+    s = fmax(s, 1e-5);
+    for (int k = 0; k < K; ++k) Q_EM[i * K + k] /= s;
+}
+
+// em_adam_gpu.py:11-42 + :222 (P side), elementwise.
+__global__ void adam_p_kernel(double* __restrict__ P, const double* __restrict__ PE, double* __restrict__ m,
+                              double* __restrict__ v, int64_t M, int K, double bc1, double bc2, double lr,
+                              double b1, double b2, double eps, float* __restrict__ P_lo, int64_t ld_lo) {
+    const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= M * K) return;
+    const double d = PE[x] - P[x];
+    const double mm = m[x] * b1 + d * (1.0 - b1);
+    const double vv = v[x] * b2 + (1.0 - b2) * d * d;
+    m[x] = mm;
+    v[x] = vv;
+    double p = P[x] + lr * (mm / bc1) / (sqrt(vv / bc2) + eps);
+    p = fmin(fmax(p, 1e-5), 1.0 - 1e-5);
+    P[x] = p;
+    if (P_lo) P_lo[(x / K) * ld_lo + (x % K)] = (float)p;
+}
+
+// em_adam_gpu.py:11-42 + :223-224 (Q side), one thread per sample.
+__global__ void adam_q_kernel(double* __restrict__ Q, const double* __restrict__ QE, double* __restrict__ m,
+                              double* __restrict__ v, int64_t N, int K, double bc1, double bc2, double lr,
+                              double b1, double b2, double eps, float* __restrict__ Q_lo, int64_t ld_lo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const int64_t x = i * K + k;
+        const double d = QE[x] - Q[x];
+        const double mm = m[x] * b1 + d * (1.0 - b1);
+        const double vv = v[x] * b2 + (1.0 - b2) * d * d;
+        m[x] = mm;
+        v[x] = vv;
+        double q = Q[x] + lr * (mm / bc1) / (sqrt(vv / bc2) + eps);
+        q = fmin(fmax(q, 1e-5), 1.0 - 1e-5);
+        Q[x] = q;
+        s += q;
+    }
+    for (int k = 0; k < K; ++k) {
+        const double q = Q[i * K + k] / s;
+        Q[i * K + k] = q;
+        if (Q_lo) Q_lo[i * ld_lo + k] = (float)q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side dispatch
+// ---------------------------------------------------------------------------
+struct EmGrid {
+    int64_t nbx, splits, n_per_split;
+};
+
+template <typename T>
+constexpr int em_cw() {
+    return sizeof(T) == 4 ? 8 : 4;
+}
+
+static EmGrid em_grid(int dtype, int64_t M, int64_t N) {
+    const int cw = dtype == ADX_F32 ? em_cw<float>() : em_cw<double>();
+    EmGrid gd;
+    gd.nbx = ceil_div(M, 64 * cw);
+    const int64_t target = 2048;
+    int64_t s = ceil_div(target, gd.nbx);
+    const int64_t max_s = ceil_div(N, 64);
+    if (s > max_s) s = max_s;
+    if (s < 1) s = 1;
+    if (s > 65535) s = 65535;
+    gd.n_per_split = ceil_div(ceil_div(N, s), 64) * 64;
+    gd.splits = ceil_div(N, gd.n_per_split);
+    return gd;
+}
+
+template <typename T, int KS, bool STATS, bool LL>
+static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
+                      int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B, T* Tm,
+                      double* llp) {
+    dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
+    em_pass_kernel<T, KS, em_cw<T>(), STATS, LL>
+        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp);
+}
+
+template <typename T, bool STATS, bool LL>
+static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M,
+                        int64_t N, int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B,
+                        T* Tm, double* llp) {
+    switch ((K + 3) / 4) {
+        case 1: launch_em<T, 1, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
+        case 2: launch_em<T, 2, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
+        case 3: launch_em<T, 3, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
+        default: launch_em<T, 4, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
+    }
+}
+
+}  // namespace adx
+
+using namespace adx;
+
+extern "C" {
+
+int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
+    (void)K;
+    if (M <= 0 || N <= 0) return 0;
+    const EmGrid gd = em_grid(dtype, M, N);
+    return (int64_t)sizeof(double) * gd.nbx * gd.splits;
+}
+
+int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
+                int64_t ldp, const void* Q, int64_t ldq, void* A, void* B, void* T, int flags,
+                double* ll_out, void* workspace, int64_t workspace_bytes, void* stream) {
+    if (M <= 0 || N <= 0) return fail_msg(1, "adx_em_pass: empty genotype matrix");
+    if (K < 1 || K > 16) return fail_msg(1, "adx_em_pass: K must be in [1, 16]");
+    if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64)
+        return fail_msg(1, "adx_em_pass: ldg must be a multiple of 16 and >= N rounded up to 64");
+    if (ldp < K || ldq < K) return fail_msg(1, "adx_em_pass: ldp/ldq < K");
+    if (reinterpret_cast<uintptr_t>(G) % 16 != 0) return fail_msg(1, "adx_em_pass: G not 16-byte aligned");
+    const bool stats = flags & ADX_EM_STATS;
+    const bool ll = flags & ADX_EM_LOGLIK;
+    if (!stats && !ll) return 0;
+    if (ll && dtype != ADX_F64) return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ADX_F64");
+    if (stats && (!A || !B || !T)) return fail_msg(1, "adx_em_pass: A, B, T required");
+    const EmGrid gd = em_grid(dtype, M, N);
+    double* llp = nullptr;
+    if (ll) {
+        const int64_t need = (int64_t)sizeof(double) * gd.nbx * gd.splits;
+        if (!ll_out || !workspace || workspace_bytes < need)
+            return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ll_out and adx_em_workspace_bytes of workspace");
+        llp = static_cast<double*>(workspace);
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == ADX_F32) {
+        launch_em_k<float, true, false>(gd, st, G, ldg, M, N, K, (const float*)P, ldp, (const float*)Q, ldq,
+                                        (float*)A, (float*)B, (float*)T, nullptr);
+    } else if (dtype == ADX_F64) {
+        const double* Pd = (const double*)P;
+        const double* Qd = (const double*)Q;
+        if (stats && ll)
+            launch_em_k<double, true, true>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, (double*)A, (double*)B,
+                                            (double*)T, llp);
+        else if (stats)
+            launch_em_k<double, true, false>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, (double*)A, (double*)B,
+                                             (double*)T, nullptr);
+        else
+            launch_em_k<double, false, true>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, nullptr, nullptr,
+                                             nullptr, llp);
+    } else {
+        return fail_msg(1, "adx_em_pass: dtype must be ADX_F32 or ADX_F64");
+    }
+    ADX_LAUNCH_CHECK("adx_em_pass");
+    if (ll) {
+        reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, ll_out);
+        ADX_LAUNCH_CHECK("adx_em_pass(reduce)");
+    }
+    return 0;
+}
+
+int adx_em_finalize(int acc_dtype, const double* P, const double* Q, void* A, void* B, void* T,
+                    const double* qbat, int64_t M, int64_t N, int K, double* P_EM, double* Q_EM, void* stream) {
+    if (K < 1) return fail_msg(1, "adx_em_finalize: K < 1");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = M * K;
+    if (acc_dtype == ADX_F32) {
+        if (n > 0) finalize_p_kernel<float><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(P, (float*)A, (float*)B, n, P_EM);
+        if (N > 0) finalize_q_kernel<float><<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(Q, (float*)T, qbat, N, K, Q_EM);
+    } else if (acc_dtype == ADX_F64) {
+        if (n > 0) finalize_p_kernel<double><<<(unsigned)ceil_div(n, 256), 256, 0, st>>>(P, (double*)A, (double*)B, n, P_EM);
+        if (N > 0) finalize_q_kernel<double><<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(Q, (double*)T, qbat, N, K, Q_EM);
+    } else {
+        return fail_msg(1, "adx_em_finalize: acc_dtype must be ADX_F32 or ADX_F64");
+    }
+    ADX_LAUNCH_CHECK("adx_em_finalize");
+    return 0;
+}
+
+int adx_adam_step(double* P, double* Q, const double* P_EM, const double* Q_EM, double* mP, double* vP,
+                  double* mQ, double* vQ, int64_t M, int64_t N, int K, double t, double lr, double beta1,
+                  double beta2, double eps, float* P_lo, float* Q_lo, int64_t ld_lo, void* stream) {
+    if (K < 1) return fail_msg(1, "adx_adam_step: K < 1");
+    if ((P_lo || Q_lo) && ld_lo < K) return fail_msg(1, "adx_adam_step: ld_lo < K");
+    const double bc1 = 1.0 - pow(beta1, t);
+    const double bc2 = 1.0 - pow(beta2, t);
+    hipStream_t st = (hipStream_t)stream;
+    if (M > 0)
+        adam_p_kernel<<<(unsigned)ceil_div(M * K, 256), 256, 0, st>>>(P, P_EM, mP, vP, M, K, bc1, bc2, lr, beta1,
+                                                                     beta2, eps, P_lo, ld_lo);
+    if (N > 0)
+        adam_q_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(Q, Q_EM, mQ, vQ, N, K, bc1, bc2, lr, beta1, beta2,
+                                                                  eps, Q_lo, ld_lo);
+    ADX_LAUNCH_CHECK("adx_adam_step");
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,241 @@
+"""Adam-EM on the GPU through libadx (drop-in for adamixture/model/em_adam_gpu.py).
+
+Signatures follow the reference module; the chunk/threads arguments are
+accepted for compatibility but the work is one fused kernel launch per EM
+pass (decode + H = P Q^T + G/H terms + A/B/T reductions), so no genotype
+chunk is ever materialised.
+
+Precision forms (``precision``):
+  * ``"f64"`` (default): the reference's arithmetic (utils.py:235-250 returns
+    float64 on CUDA); H and the A/B/T contractions on FP64 MFMA.
+  * ``"f32"``: H and the contractions on FP32 MFMA from FP32 copies of P and
+    Q; P, Q, the Adam state and the log-likelihood stay FP64.
+"""
+from __future__ import annotations
+
+import logging
+import os
+import time
+
+import torch
+
+from . import _lib
+from .genotypes import PackedGenotypes, as_packed
+
+log = logging.getLogger(__name__)
+
+PRECISIONS = ("f64", "f32")
+
+
+def default_precision() -> str:
+    p = os.environ.get("ADX_PRECISION", "f64")
+    if p not in PRECISIONS:
+        raise ValueError(f"ADX_PRECISION must be one of {PRECISIONS}")
+    return p
+
+
+def _check_params(P: torch.Tensor, Q: torch.Tensor) -> None:
+    if P.dtype != torch.float64 or Q.dtype != torch.float64:
+        raise TypeError("P and Q must be float64 tensors")
+    if not (P.is_cuda and Q.is_cuda and P.is_contiguous() and Q.is_contiguous()):
+        raise ValueError("P and Q must be contiguous CUDA tensors")
+    if P.shape[1] != Q.shape[1]:
+        raise ValueError("P and Q must have the same number of columns K")
+
+
+class _Workspace:
+    """Per-device scratch for the log-likelihood partial sums."""
+
+    def __init__(self):
+        self.buf: torch.Tensor | None = None
+        self.ll = None
+
+    def get(self, device, nbytes: int) -> tuple[torch.Tensor, torch.Tensor]:
+        if self.buf is None or self.buf.numel() < nbytes or self.buf.device != device:
+            self.buf = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=device)
+            self.ll = torch.zeros(1, dtype=torch.float64, device=device)
+        return self.buf, self.ll
+
+
+_WS = _Workspace()
+
+
+def em_pass(G: PackedGenotypes, P, Q, A=None, B=None, T=None, *, dtype: int = _lib.ADX_F64,
+            loglik: bool = False, stats: bool = True):
+    """One fused pass.  Returns the log-likelihood tensor (1 element, FP64,
+    on device) when ``loglik`` is set, else None."""
+    K = P.shape[1]
+    flags = (_lib.ADX_EM_STATS if stats else 0) | (_lib.ADX_EM_LOGLIK if loglik else 0)
+    ws_bytes = _lib.load().adx_em_workspace_bytes(dtype, G.M, G.N, K)
+    ws, ll = _WS.get(G.device, ws_bytes) if loglik else (None, None)
+    _lib.call("adx_em_pass", dtype, G.data.data_ptr(), G.ldg, G.M, G.N, K, P.data_ptr(), P.stride(0),
+              Q.data_ptr(), Q.stride(0), _lib.ptr(A), _lib.ptr(B), _lib.ptr(T), flags, _lib.ptr(ll),
+              _lib.ptr(ws), ws_bytes if loglik else 0, _lib.stream_handle(G.device))
+    return ll
+
+
+class EMAdamOptimizer:
+    """em_adam_gpu.py:121-224: Adam state, EM accumulators and targets."""
+
+    def __init__(self, P_shape, Q_shape, lr: float, beta1: float, beta2: float, reg_adam: float,
+                 device: torch.device, precision: str | None = None) -> None:
+        self.device = torch.device(device)
+        self.lr, self.beta1, self.beta2, self.reg_adam = lr, beta1, beta2, reg_adam
+        self.precision = precision or default_precision()
+        if self.precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}")
+        self.dtype = torch.float64
+        self.t = 0.0
+        M, K = P_shape
+        N = Q_shape[0]
+        z = lambda *s, dt=torch.float64: torch.zeros(*s, dtype=dt, device=self.device)  # noqa: E731
+        self.m_P, self.v_P, self.m_Q, self.v_Q = z(M, K), z(M, K), z(N, K), z(N, K)
+        acc_dt = torch.float32 if self.precision == "f32" else torch.float64
+        self.acc_dtype = _lib.ADX_F32 if self.precision == "f32" else _lib.ADX_F64
+        self.A_accum, self.B_accum = z(M, K, dt=acc_dt), z(M, K, dt=acc_dt)
+        self.T_accum = z(N, K, dt=acc_dt)
+        self.P_EM, self.Q_EM = z(M, K), z(N, K)
+        if self.precision == "f32":
+            self.P_lo = torch.zeros(M, K, dtype=torch.float32, device=self.device)
+            self.Q_lo = torch.zeros(N, K, dtype=torch.float32, device=self.device)
+
+    def accumulate(self, G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, loglik: bool = False):
+        """EM statistics at (P, Q) into A/B/T; optionally the FP64 logL at the
+        same point (fused into the same pass in the f64 form)."""
+        if self.precision == "f64":
+            return em_pass(G, P, Q, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F64, loglik=loglik)
+        self.P_lo.copy_(P)
+        self.Q_lo.copy_(Q)
+        em_pass(G, self.P_lo, self.Q_lo, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F32)
+        return em_pass(G, P, Q, dtype=_lib.ADX_F64, loglik=True, stats=False) if loglik else None
+
+    def finalize(self, G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor):
+        _lib.call("adx_em_finalize", self.acc_dtype, P.data_ptr(), Q.data_ptr(), self.A_accum.data_ptr(),
+                  self.B_accum.data_ptr(), self.T_accum.data_ptr(), G.qbat().data_ptr(), P.shape[0],
+                  Q.shape[0], P.shape[1], self.P_EM.data_ptr(), self.Q_EM.data_ptr(),
+                  _lib.stream_handle(self.device))
+        return self.P_EM, self.Q_EM
+
+    def run_em_step(self, G, P: torch.Tensor, Q: torch.Tensor, M: int, chunk_size: int = 0,
+                    unpacker=None) -> tuple[torch.Tensor, torch.Tensor]:
+        """em_adam_gpu.py:166-199: one full EM sweep -> (P_EM, Q_EM)."""
+        del chunk_size, unpacker
+        _check_params(P, Q)
+        G = as_packed(G, M, self.device)
+        self.accumulate(G, P, Q)
+        return self.finalize(G, P, Q)
+
+    def step(self, P: torch.Tensor, Q: torch.Tensor, P_target: torch.Tensor, Q_target: torch.Tensor) -> None:
+        """em_adam_gpu.py:201-224: Adam toward the targets, clamp, renormalise."""
+        _check_params(P, Q)
+        self.t += 1.0
+        _lib.call("adx_adam_step", P.data_ptr(), Q.data_ptr(), P_target.contiguous().data_ptr(),
+                  Q_target.contiguous().data_ptr(), self.m_P.data_ptr(), self.v_P.data_ptr(),
+                  self.m_Q.data_ptr(), self.v_Q.data_ptr(), P.shape[0], Q.shape[0], P.shape[1], self.t,
+                  self.lr, self.beta1, self.beta2, self.reg_adam, None, None, 0,
+                  _lib.stream_handle(self.device))
+
+
+def loglikelihood_gpu(G, P: torch.Tensor, Q: torch.Tensor, M: int, N: int, batch_size: int = 0,
+                      device: torch.device | None = None, threads_per_block: int = 0) -> float:
+    """em_adam_gpu.py:250-279: FP64 log-likelihood, one fused pass."""
+    del N, batch_size, threads_per_block
+    P64 = P.to(dtype=torch.float64).contiguous()
+    Q64 = Q.to(device=P64.device, dtype=torch.float64).contiguous()
+    G = as_packed(G, M, device or P64.device)
+    return em_pass(G, P64, Q64, loglik=True, stats=False).item()
+
+
+def _optimize_parameters_gpu_once(G, P, Q, lr, beta1, beta2, reg_adam, max_iter, check, K, M, N, lr_decay,
+                                  min_lr, patience, tol_adam, device, chunk_size, threads_per_block,
+                                  precision=None, on_check=None):
+    """em_adam_gpu.py:281-356 with the log-likelihood of each check fused into
+    the next EM pass (which runs at the same (P, Q)): the decision taken at
+    check iteration ``it`` happens before the Adam step of iteration it+1,
+    so the trajectory, the lr schedule and the returned best (P, Q) are the
+    reference's; at most one EM pass is computed and discarded at the end.
+    ``on_check(it, logl)`` is called at every evaluation."""
+    del K, N, chunk_size, threads_per_block
+    device = torch.device(device)
+    G = as_packed(G, M, device)
+    opt = EMAdamOptimizer(P.shape, Q.shape, lr, beta1, beta2, reg_adam, device, precision)
+    ts = time.time()
+    opt.accumulate(G, P, Q)
+    opt.finalize(G, P, Q)
+    opt.step(P, Q, opt.P_EM, opt.Q_EM)
+    log.info(f"    Performed priming iteration... ({time.time() - ts:.1f}s)\n")
+
+    P_best = P.clone()
+    Q_best = Q.clone()
+    L_best = None
+    wait_lr = 0
+    pending = 0          # iteration index whose logL the next pass evaluates (0 = priming)
+    ts = time.time()
+
+    def decide(it_done: int, L_cur: float) -> bool:
+        """Checkpoint logic of em_adam_gpu.py:334-353; True -> stop."""
+        nonlocal L_best, wait_lr
+        if on_check is not None:
+            on_check(it_done, L_cur)
+        if L_best is None:            # the post-priming evaluation (:325-327)
+            L_best = L_cur
+            return False
+        log.info(f"    Iteration {it_done}, Log-likelihood: {L_cur:.1f}")
+        if L_cur > L_best + tol_adam:
+            L_best = L_cur
+            P_best.copy_(P)
+            Q_best.copy_(Q)
+            wait_lr = 0
+        else:
+            wait_lr += 1
+            if wait_lr >= patience:
+                old = opt.lr
+                opt.lr = max(opt.lr * lr_decay, min_lr)
+                log.info(f"    Plateau reached ({wait_lr} checks without beating best). "
+                         f"Reducing lr: {old:.3e} → {opt.lr:.3e}")
+                if opt.lr <= min_lr:
+                    log.info("        Convergence reached.")
+                    return True
+                wait_lr = 0
+        return False
+
+    stopped = False
+    for it in range(max_iter):
+        ll = opt.accumulate(G, P, Q, loglik=pending is not None)
+        if pending is not None:
+            if decide(pending, ll.item()):
+                stopped = True
+                break
+            pending = None
+        opt.finalize(G, P, Q)
+        opt.step(P, Q, opt.P_EM, opt.Q_EM)
+        if (it + 1) % check == 0:
+            pending = it + 1
+    if not stopped and pending is not None:
+        ll = em_pass(G, P, Q, loglik=True, stats=False)
+        decide(pending, ll.item())
+    log.info(f"\n    Final log-likelihood: {L_best:.1f}")
+    return P_best, Q_best
+
+
+def optimize_parameters_gpu(G, P, Q, lr, beta1, beta2, reg_adam, max_iter, check, K, M, N, lr_decay, min_lr,
+                            patience, tol_adam, device, chunk_size, threads_per_block, precision=None,
+                            on_check=None):
+    """em_adam_gpu.py:359-385: Adam-EM with the reference's OOM retry (the
+    retry restarts from the unmodified P, Q; the fused pass has no chunk
+    temporaries, so the halved chunk size changes nothing but is kept)."""
+    from .utils import MIN_GPU_CHUNK_SIZE, is_gpu_oom, reduce_gpu_chunk_or_raise
+    cur = chunk_size
+    P_attempt = torch.empty_like(P)
+    Q_attempt = torch.empty_like(Q)
+    while True:
+        P_attempt.copy_(P)
+        Q_attempt.copy_(Q)
+        try:
+            return _optimize_parameters_gpu_once(G, P_attempt, Q_attempt, lr, beta1, beta2, reg_adam, max_iter,
+                                                 check, K, M, N, lr_decay, min_lr, patience, tol_adam, device,
+                                                 cur, threads_per_block, precision, on_check)
+        except RuntimeError as exc:
+            if not is_gpu_oom(exc) or cur <= MIN_GPU_CHUNK_SIZE:
+                raise
+            cur = reduce_gpu_chunk_or_raise(cur, exc, "running Adam-EM")

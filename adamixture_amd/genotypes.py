@@ -1,0 +1,143 @@
+"""Device-resident SNP-packed genotypes and the genotype-level GPU ops.
+
+Layout (the reference's, adamixture/src/utils_c/cython/snp_reader.pyx:76-120):
+``ceil(M/4)`` byte rows, byte ``[j, i]`` holds SNPs ``4j..4j+3`` of sample
+``i`` at bits ``2k..2k+1``; codes 0/1/2 = allele count, 3 = missing.  On the
+device every row is padded to ``ldg = roundup(N, 64)`` bytes (padding bytes
+0xFF, i.e. missing) so the EM kernels can stream 16-byte aligned 64-sample
+sub-tiles without bounds checks.  At N = 100,000, M = 500,000 the matrix is
+12.5 GB, resident in HBM.
+"""
+from __future__ import annotations
+
+import weakref
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def padded_ld(N: int) -> int:
+    return ((N + 63) // 64) * 64
+
+
+class PackedGenotypes:
+    """SNP-packed genotype matrix on one GPU with 64-padded rows."""
+
+    def __init__(self, data: torch.Tensor, M: int, N: int):
+        if data.dtype != torch.uint8 or data.dim() != 2 or not data.is_cuda:
+            raise ValueError("PackedGenotypes needs a 2-D uint8 CUDA tensor")
+        if data.shape[0] != (M + 3) // 4 or data.shape[1] < N or data.stride(1) != 1:
+            raise ValueError("PackedGenotypes: data must be ceil(M/4) x ldg with ldg >= N")
+        if data.stride(0) % 16 != 0 or data.data_ptr() % 16 != 0:
+            raise ValueError("PackedGenotypes: rows must be 16-byte aligned")
+        self.data = data
+        self.M = int(M)
+        self.N = int(N)
+        self.ldg = int(data.stride(0))
+        self._qbat: torch.Tensor | None = None
+
+    # The reference's helpers test packedness through the shape
+    # (adamixture/src/utils.py:483-494); expose the logical shape.
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self.data.shape[0], self.N)
+
+    @property
+    def device(self) -> torch.device:
+        return self.data.device
+
+    def qbat(self) -> torch.Tensor:
+        """2 x observed genotypes per sample (FP64), computed once."""
+        if self._qbat is None:
+            q = torch.empty(self.N, dtype=torch.float64, device=self.device)
+            _lib.call("adx_qbat", self.data.data_ptr(), self.ldg, self.M, self.N, q.data_ptr(),
+                      _lib.stream_handle(self.device))
+            self._qbat = q
+        return self._qbat
+
+    def to_host_packed(self) -> np.ndarray:
+        """Unpadded ceil(M/4) x N numpy copy."""
+        return self.data[:, : self.N].cpu().numpy().copy()
+
+
+# Device copies keyed by the source tensor (dropped when the source dies).
+_CACHE: "weakref.WeakKeyDictionary[torch.Tensor, dict]" = weakref.WeakKeyDictionary()
+
+
+def as_packed(G, M: int, device: torch.device | str | None = None) -> PackedGenotypes:
+    """Return ``G`` as PackedGenotypes on ``device``.
+
+    Accepts PackedGenotypes, or a ceil(M/4) x N uint8 numpy array / torch
+    tensor (host or device) as produced by the reference reader.  Device
+    copies of a given tensor are cached so repeated calls through the
+    reference-style API do not repeat the upload.
+    """
+    if isinstance(G, PackedGenotypes):
+        return G
+    if isinstance(G, np.ndarray):
+        G = torch.from_numpy(np.ascontiguousarray(G))
+    if not isinstance(G, torch.Tensor) or G.dtype != torch.uint8 or G.dim() != 2:
+        raise TypeError("genotypes must be a 2-D uint8 packed matrix")
+    Mb, N = G.shape
+    if Mb != (M + 3) // 4:
+        raise ValueError(f"packed genotypes have {Mb} byte rows, expected ceil({M}/4)")
+    dev = torch.device(device) if device is not None else (G.device if G.is_cuda else torch.device("cuda"))
+    per_src = _CACHE.setdefault(G, {})
+    key = (G.data_ptr(), str(dev), M, G._version)
+    hit = per_src.get(key)
+    if hit is not None:
+        return hit
+    ldg = padded_ld(N)
+    out = torch.empty((Mb, ldg), dtype=torch.uint8, device=dev)
+    src = G if G.is_cuda and G.device == dev else G.to(dev)
+    if src.stride(1) != 1:
+        src = src.contiguous()
+    _lib.call("adx_repack_padded", src.data_ptr(), src.stride(0), Mb, N, out.data_ptr(), ldg,
+              _lib.stream_handle(dev))
+    pg = PackedGenotypes(out, M, N)
+    per_src.clear()
+    per_src[key] = pg
+    return pg
+
+
+def unpack_chunk(G: PackedGenotypes, m_start: int, chunk: int) -> torch.Tensor:
+    """pack2bit.cu:83-110 semantics: codes of SNPs m_start .. m_start+C'-1."""
+    rows = max(0, min(chunk, G.M - m_start))
+    out = torch.empty((rows, G.N), dtype=torch.uint8, device=G.device)
+    if rows:
+        _lib.call("adx_unpack_u8", G.data.data_ptr(), G.ldg, G.M, G.N, m_start, rows, 0, out.data_ptr(),
+                  _lib.stream_handle(G.device))
+    return out
+
+
+def unpack_center_chunk(G: PackedGenotypes, f: torch.Tensor, m_start: int, chunk: int) -> torch.Tensor:
+    """pack2bit.cu:112-141 semantics: g - 2 f[m] as FP32, 0 where missing."""
+    rows = max(0, min(chunk, G.M - m_start))
+    out = torch.empty((rows, G.N), dtype=torch.float32, device=G.device)
+    f = f.to(device=G.device, dtype=torch.float32).contiguous()
+    if rows:
+        _lib.call("adx_unpack_center_f32", G.data.data_ptr(), G.ldg, f.data_ptr(), G.M, G.N, m_start, rows, 0,
+                  out.data_ptr(), _lib.stream_handle(G.device))
+    return out
+
+
+def allele_frequencies(G: PackedGenotypes) -> torch.Tensor:
+    """utils.py:781-829 semantics, one pass over the packed matrix (FP32)."""
+    f = torch.empty(G.M, dtype=torch.float32, device=G.device)
+    _lib.call("adx_freq_f32", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), _lib.stream_handle(G.device))
+    return f
+
+
+def synthesize(P: torch.Tensor, Q: torch.Tensor, seed: int, miss_rate: float) -> PackedGenotypes:
+    """Counter-based synthetic genotypes G ~ Binomial(2, Q P^T) on the device."""
+    P = P.to(dtype=torch.float64).contiguous()
+    Q = Q.to(device=P.device, dtype=torch.float64).contiguous()
+    M, K = P.shape
+    N = Q.shape[0]
+    ldg = padded_ld(N)
+    out = torch.empty(((M + 3) // 4, ldg), dtype=torch.uint8, device=P.device)
+    _lib.call("adx_synth_packed", P.data_ptr(), Q.data_ptr(), M, N, K, seed & 0xFFFFFFFFFFFFFFFF, miss_rate,
+              out.data_ptr(), ldg, _lib.stream_handle(P.device))
+    return PackedGenotypes(out, M, N)

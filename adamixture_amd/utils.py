@@ -1,0 +1,168 @@
+"""Device utilities with the signatures of adamixture/src/utils.py.
+
+The unpack / log-likelihood / frequency seams call libadx; the OOM and
+output helpers keep the reference's behaviour.
+"""
+from __future__ import annotations
+
+import gc
+import logging
+import random
+from collections.abc import Callable
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib
+from .genotypes import PackedGenotypes, allele_frequencies, as_packed, unpack_center_chunk, unpack_chunk
+
+log = logging.getLogger(__name__)
+
+MIN_GPU_CHUNK_SIZE = 2
+MIN_CPU_CHUNK_SIZE = 2
+
+
+def is_gpu_oom(exc: BaseException) -> bool:
+    """utils.py:19-43: OOM recognised by exception type or message."""
+    message = str(exc).lower()
+    return isinstance(exc, torch.cuda.OutOfMemoryError) or "out of memory" in message
+
+
+def reduce_gpu_chunk_or_raise(chunk_size: int, exc: BaseException, context: str) -> int:
+    """utils.py:62-89: halve the chunk after an OOM and release cached memory."""
+    if chunk_size <= MIN_GPU_CHUNK_SIZE:
+        raise exc
+    new = max(chunk_size // 2, MIN_GPU_CHUNK_SIZE)
+    exc.__traceback__ = None
+    gc.collect()
+    if torch.cuda.is_available():
+        torch.cuda.empty_cache()
+    log.warning(f"    GPU out of memory while {context}: retrying with chunk_size={new}.")
+    return new
+
+
+def get_dtype(device: torch.device) -> torch.dtype:
+    """utils.py:235-250: FP64 for the GPU path."""
+    return torch.float64
+
+
+def get_tuning_params(device: torch.device) -> int:
+    """utils.py:207-233.  Kept for signature compatibility; the fused kernels
+    pick their own launch shapes (256-thread workgroups, wave64)."""
+    return 256
+
+
+def load_extensions(device: torch.device) -> None:
+    """utils.py:431-480 JIT-compiled three CUDA extensions; here the
+    prebuilt libadx.so is loaded (raises if it is missing)."""
+    if torch.device(device).type == "cuda":
+        _lib.load()
+
+
+def is_packed_genotype_tensor(G, M: int) -> bool:
+    """utils.py:483-494."""
+    return isinstance(G, PackedGenotypes) or (isinstance(G, torch.Tensor) and G.ndim == 2 and G.size(0) != M)
+
+
+def get_unpacker(device: torch.device, threads_per_block: int = 0) -> Callable:
+    """utils.py:497-559: (G, start, C, M) -> C x N uint8 codes on the device."""
+    dev = torch.device(device)
+
+    def unpack(G, start_idx: int, actual_chunk_size: int, M: int) -> torch.Tensor:
+        return unpack_chunk(as_packed(G, M, dev), start_idx, actual_chunk_size)
+
+    return unpack
+
+
+def get_centering_unpacker(device: torch.device, threads_per_block: int = 0) -> Callable:
+    """utils.py:713-779: (G, f, start, C, M) -> C x N centred FP32 chunk."""
+    dev = torch.device(device)
+
+    def unpack_center(G, f: torch.Tensor, start_idx: int, actual_chunk_size: int, M: int) -> torch.Tensor:
+        return unpack_center_chunk(as_packed(G, M, dev), f, start_idx, actual_chunk_size)
+
+    return unpack_center
+
+
+def get_logl_calculator(device: torch.device) -> Callable:
+    """utils.py:605-658: (G, P, Q, M, N, batch, tpb) -> FP64 log-likelihood."""
+    from .em_adam_gpu import loglikelihood_gpu
+    dev = torch.device(device)
+
+    def logl(G, P, Q, M, N, batch_size=0, threads_per_block=0) -> float:
+        return loglikelihood_gpu(G, P, Q, M, N, batch_size, dev, threads_per_block)
+
+    return logl
+
+
+def calculate_frequencies_gpu(G, M: int, chunk_size: int, device_obj: torch.device,
+                              threads_per_block: int = 0) -> torch.Tensor:
+    """utils.py:832-861: per-SNP allele frequency (FP32), one fused pass."""
+    return allele_frequencies(as_packed(G, M, device_obj))
+
+
+def get_free_gpu_memory(device: torch.device) -> float:
+    """utils.py:290-308: MB available to new tensors."""
+    device = torch.device(device)
+    torch.cuda.synchronize(device)
+    free, _ = torch.cuda.mem_get_info(device)
+    return round((free + torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)) / 2**20, 2)
+
+
+def estimate_gpu_bytes(M: int, N: int, K: int, algorithm: str, q_hist: int = 3,
+                       include_initialization: bool = True) -> int:
+    """Peak device bytes besides G for the fused kernels.  Unlike the
+    reference estimate (utils.py:360-417) there are no C x N chunk
+    temporaries: only O((M + N) K) buffers."""
+    L = max(K + 10, 20)
+    svd = M * L * 4 + 2 * N * L * 4 + M * L * 4
+    als = 5 * (M + N) * K * 8
+    if algorithm == "adamem":
+        opt = (8 * M * K + 7 * N * K + N) * 8 + 2 * (M + N) * K * 4
+    elif algorithm == "brqn":
+        dim = (M + N) * K
+        opt = ((M + N) * K * K + (M + N) * K) * 8 + 3 * dim * q_hist * 8 + 9 * dim * 8
+    else:
+        raise ValueError(f"Unknown algorithm for GPU memory estimate: {algorithm}")
+    return max(svd, als, opt) if include_initialization else opt
+
+
+def manage_gpu_memory(G, device: torch.device, M: int, N: int, K: int, chunk_size: int, algorithm: str,
+                      q_hist: int = 3, include_initialization: bool = True):
+    """utils.py:311-429: place G on the device when it fits.
+
+    Returns PackedGenotypes (device-resident, padded layout).  If G and the
+    working buffers do not fit in 95% of free memory a RuntimeError naming
+    "out of memory" is raised (host-streamed EM is not implemented)."""
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        return G
+    if isinstance(G, PackedGenotypes):
+        return G
+    need = ((M + 3) // 4) * ((N + 63) // 64 * 64) + estimate_gpu_bytes(M, N, K, algorithm, q_hist,
+                                                                      include_initialization)
+    free_mb = get_free_gpu_memory(dev)
+    if free_mb * 0.95 * 2**20 < need:
+        raise RuntimeError(f"out of memory: genotypes and buffers need {need / 2**30:.1f} GiB, "
+                           f"{free_mb / 1024:.1f} GiB free")
+    log.info("    Moving genotype matrix to GPU...")
+    return as_packed(G, M, dev)
+
+
+def write_outputs(Q: np.ndarray, run_name: str, K: int, out_path, P: np.ndarray | None = None) -> None:
+    """utils.py:252-273: ADMIXTURE-compatible .Q / .P text files
+    (numpy.savetxt defaults: '%.18e', space-delimited)."""
+    out_path = Path(out_path)
+    np.savetxt(out_path / f"{run_name}.{K}.Q", Q, delimiter=" ")
+    if P is not None:
+        np.savetxt(out_path / f"{run_name}.{K}.P", P, delimiter=" ")
+
+
+def set_seed(seed: int) -> None:
+    """utils.py:276-287."""
+    np.random.seed(seed)
+    random.seed(seed)
+    torch.manual_seed(seed)
+    if torch.cuda.is_available():
+        torch.cuda.manual_seed_all(seed)

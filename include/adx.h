@@ -1,0 +1,121 @@
+/*
+ * adx.h -- C ABI of libadx, the gfx950 (MI355X) kernels behind the
+ * `--device gpu` likelihood path of ADAMIXTURE.
+ *
+ * Conventions
+ *   - Plain device pointers and sizes; no torch types.  `stream` is a
+ *     hipStream_t passed as void* (NULL = default stream).  All entry points
+ *     are asynchronous on that stream unless stated otherwise.
+ *   - Return 0 on success, non-zero on failure; adx_last_error() then holds a
+ *     message for the calling thread.  HIP out-of-memory failures carry the
+ *     text "out of memory" (the reference classifies OOM by message,
+ *     adamixture/src/utils.py:19-43).
+ *   - Genotypes use the reference's device layout (adamixture/src/utils_c/
+ *     cython/snp_reader.pyx:76-120): Mb = ceil(M/4) byte rows of `ldg` bytes,
+ *     byte [j, i] holds SNPs 4j..4j+3 of sample i at bits 2k..2k+1; codes
+ *     0/1/2 = allele count, 3 = missing.  Entry points that stream whole
+ *     rows need ldg % 16 == 0; bytes i >= N of a row are padding.
+ *   - Matrices are row-major: P is M x K (stride ldp), Q is N x K (stride ldq).
+ *   - dtype tags: ADX_F32 = 0, ADX_F64 = 1, ADX_BF16 = 2.
+ *
+ * Reference citations are paths relative to the reference repository root.
+ */
+#ifndef ADX_H
+#define ADX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ADX_F32 = 0, ADX_F64 = 1, ADX_BF16 = 2 };
+
+/* Flags of adx_em_pass. */
+enum { ADX_EM_STATS = 1, ADX_EM_LOGLIK = 2 };
+
+const char *adx_last_error(void);
+
+/* --- genotype layout ----------------------------------------------------- */
+
+/* Replaces torch.ops.pack2bit.unpack2bit_gpu_chunk_uint8
+ * (adamixture/src/utils_c/cuda/pack2bit.cu:83-110, registered at :154):
+ * out (C' x N, uint8) = codes of SNPs m_start .. m_start+C'-1, with
+ * C' = min(C, M - m_start).  byte_offset = first byte row present in G. */
+int adx_unpack_u8(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int64_t m_start, int64_t C,
+                  int64_t byte_offset, uint8_t *out, void *stream);
+
+/* Replaces torch.ops.pack2bit.unpack2bit_gpu_chunk_center
+ * (pack2bit.cu:112-141, registered at :153): out = g - 2 f[m], 0 if missing. */
+int adx_unpack_center_f32(const uint8_t *G, int64_t ldg, const float *f, int64_t M, int64_t N,
+                          int64_t m_start, int64_t C, int64_t byte_offset, float *out, void *stream);
+
+/* Replaces freq_batch_math + _calculate_frequencies_gpu_once
+ * (adamixture/src/utils.py:781-829): f[m] = sum g / (2 #observed), 0 if none. */
+int adx_freq_f32(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, float *f, void *stream);
+
+/* Replaces the q_bat accumulation of em_batch_math
+ * (adamixture/model/em_adam_gpu.py:76, summed at :197): qbat[i] =
+ * 2 x #observed genotypes of sample i (FP64, exact). */
+int adx_qbat(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, double *qbat, void *stream);
+
+/* No reference counterpart (layout): copy Mb rows of N bytes from a tensor
+ * with row stride ld_src into rows of ldg bytes, padding bytes set to 0xFF. */
+int adx_repack_padded(const uint8_t *src, int64_t ld_src, int64_t Mb, int64_t N, uint8_t *dst,
+                      int64_t ldg, void *stream);
+
+/* No reference counterpart (synthetic workloads): G_ij ~ Binomial(2, Q_i.P_j)
+ * with missing rate miss_rate, counter-based on (seed, SNP, sample); P (M x K)
+ * and Q (N x K) are FP64 device arrays.  Padding bytes are set to 0xFF. */
+int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int K, uint64_t seed,
+                     double miss_rate, uint8_t *G, int64_t ldg, void *stream);
+
+/* --- Adam-EM -------------------------------------------------------------- */
+
+/* Workspace (bytes) adx_em_pass needs for the given shape and dtype. */
+int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
+
+/* Replaces the chunk loop of EMAdamOptimizer.run_em_step with
+ * em_batch_math (adamixture/model/em_adam_gpu.py:166-197, :46-78), and, with
+ * ADX_EM_LOGLIK, logl_batch_math (:226-248) at the same (P, Q).
+ * One pass over all genotypes; 1 <= K <= 16.
+ *   ADX_EM_STATS:  A += a Q, B += b Q (M x K), T += (a-b)^T P + colsum(b)
+ *                  (N x K), with H = P Q^T clamped to [1e-5, 1-1e-5],
+ *                  a = g/H, b = (2-g)/(1-H) on observed genotypes.  A, B, T
+ *                  are accumulated (caller zeroes them; adx_em_finalize does).
+ *   ADX_EM_LOGLIK: *ll_out (device, FP64) = sum g log H + (2-g) log1p(-H),
+ *                  H clamped to [1e-10, 1-1e-10]; FP64 dtype only.
+ * dtype selects the arithmetic of P, Q, A, B, T: ADX_F64 or ADX_F32. */
+int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K,
+                const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
+                int flags, double *ll_out, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Replaces em_final_update (adamixture/model/em_adam_gpu.py:82-117):
+ * P_EM = A P / max((A-B) P + B, 1e-5); Q_EM = Q T / max(qbat, 1e-5), rows
+ * divided by max(row sum, 1e-5).  A, B, T have dtype acc_dtype (F32/F64) and
+ * are zeroed on return.  P, Q, qbat, P_EM, Q_EM are FP64, contiguous. */
+int adx_em_finalize(int acc_dtype, const double *P, const double *Q, void *A, void *B, void *T,
+                    const double *qbat, int64_t M, int64_t N, int K, double *P_EM, double *Q_EM,
+                    void *stream);
+
+/* Replaces EMAdamOptimizer.step with adam_update (em_adam_gpu.py:201-224,
+ * :11-42): Adam with bias correction at step t (already incremented) toward
+ * the EM targets, then P, Q clamped to [1e-5, 1-1e-5] and Q rows renormalised.
+ * If P_lo / Q_lo are non-NULL, FP32 copies (row stride ld_lo) of the updated
+ * P and Q are written, zero-padded to ld_lo columns. */
+int adx_adam_step(double *P, double *Q, const double *P_EM, const double *Q_EM, double *mP, double *vP,
+                  double *mQ, double *vQ, int64_t M, int64_t N, int K, double t, double lr, double beta1,
+                  double beta2, double eps, float *P_lo, float *Q_lo, int64_t ld_lo, void *stream);
+
+/* --- tests ----------------------------------------------------------------- */
+
+/* No reference counterpart: runs one 16x16x4 MFMA (kind 0 = FP32, 1 = FP64)
+ * on row-major A (16x4) and B (4x16) with the lane mapping the EM kernels
+ * assume, writing D (16x16) row-major.  Used to pin that mapping. */
+int adx_probe_mfma(int kind, const void *A, const void *B, void *D, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADX_H */

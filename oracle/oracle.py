@@ -30,7 +30,7 @@ _ptr = ctypes.c_void_p
 def build(force: bool = False) -> None:
     """Compile the oracle shared objects with the committed Makefile."""
     if force or not (_BUILD / "liboracle.so").exists() or not (_BUILD / "liboracle_fast.so").exists():
-        subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+        subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", str(_HERE)], check=True)
 
 
 def _load(name: str) -> ctypes.CDLL:

@@ -1,0 +1,73 @@
+"""Adam-EM trajectories on the GPU against the oracle and the reference's own
+fixtures (adamixture/demo/outputs/gpu/adamem, CUDA FP64 run)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from adamixture_amd import genotypes as GT
+from adamixture_amd import reader
+from adamixture_amd.em_adam_gpu import loglikelihood_gpu, optimize_parameters_gpu
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+HP = dict(lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, check=5, lr_decay=0.5, min_lr=1e-4, patience=3,
+          tol_adam=0.1)
+
+
+def _run(G, P0, Q0, M, N, K, dev, precision, max_iter=10000):
+    hist = []
+    P, Q = optimize_parameters_gpu(G, torch.from_numpy(P0).to(dev), torch.from_numpy(Q0).to(dev),
+                                   HP["lr"], HP["beta1"], HP["beta2"], HP["reg_adam"], max_iter, HP["check"], K, M,
+                                   N, HP["lr_decay"], HP["min_lr"], HP["patience"], HP["tol_adam"], dev, 8192, 256,
+                                   precision=precision, on_check=lambda it, L: hist.append((it, L)))
+    return P.cpu().numpy(), Q.cpu().numpy(), hist
+
+
+def test_demo_adamem_f64_matches_oracle_and_reference_fixture(cuda, fixtures):
+    Gt, N, M = reader.read_bed_packed(GOLDEN / "demo_data.bed")
+    K = 7
+    P0, Q0 = fixtures["gpu_adamem_als_P"], fixtures["gpu_adamem_als_Q"]
+    P, Q, hist = _run(Gt, P0, Q0, M, N, K, cuda, "f64")
+    ohist = []
+    Po, Qo = O.adamem_gpu(fixtures["reader_G_packed"], P0, Q0, M, history=ohist)
+    print("checks gpu/oracle", len(hist), len(ohist), "max|dP|", np.abs(P - Po).max(), "max|dQ|",
+          np.abs(Q - Qo).max(), "last logL", hist[-1], ohist[-1])
+    assert [it for it, _ in hist] == [it for it, _ in ohist]      # same lr schedule and stop
+    np.testing.assert_allclose([L for _, L in hist], [L for _, L in ohist], rtol=1e-11)
+    np.testing.assert_allclose(P, Po, atol=1e-8)
+    np.testing.assert_allclose(Q, Qo, atol=1e-8)
+    # and the reference's own CUDA fixture at its tolerance (tests/config.py:24-26)
+    Pe, Qe = fixtures["gpu_adamem_adamem_P"], fixtures["gpu_adamem_adamem_Q"]
+    np.testing.assert_allclose(P, Pe, rtol=2e-2, atol=2e-2)
+    np.testing.assert_allclose(Q, Qe, rtol=2e-2, atol=2e-2)
+    G = fixtures["reader_G"]
+    assert abs(O.loglik_cpu(G, P, Q) - O.loglik_cpu(G, Pe, Qe)) <= 50.0
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_cpu_ref_workload_fixed_iterations(cuda, precision):
+    """N=1,000 x M=20,000, K=3, 1% missing: 100 Adam-EM iterations from the
+    same init on the GPU and on the oracle (FP64)."""
+    from adamixture_amd.synth import WORKLOADS, draw_params, make_genotypes
+    w = WORKLOADS["cpu_ref"]
+    Ptrue, Qtrue = draw_params(w)
+    G, _, _ = make_genotypes(w, cuda, params=(Ptrue, Qtrue))
+    rng = np.random.default_rng(w.seed)
+    P0 = np.clip(rng.random((w.M, w.K)), 1e-5, 1 - 1e-5)
+    Q0 = rng.dirichlet(np.ones(w.K), size=w.N)
+    P, Q, hist = _run(G, P0, Q0, w.M, w.N, w.K, cuda, precision, max_iter=100)
+    ohist = []
+    Po, Qo = O.adamem_gpu(G.to_host_packed(), P0, Q0, w.M, max_iter=100, history=ohist)
+    dP, dQ = np.abs(P - Po).max(), np.abs(Q - Qo).max()
+    rel_ll = abs(hist[-1][1] - ohist[-1][1]) / abs(ohist[-1][1])
+    print(f"{precision}: max|dP| {dP:.3e} max|dQ| {dQ:.3e} logL rel {rel_ll:.3e}")
+    if precision == "f64":
+        assert [it for it, _ in hist] == [it for it, _ in ohist]
+        assert dP < 1e-9 and dQ < 1e-9 and rel_ll < 1e-12
+    else:
+        assert dP < 5e-3 and dQ < 5e-3 and rel_ll < 1e-6
+    Gl = G
+    assert loglikelihood_gpu(Gl, torch.from_numpy(Po).to(cuda), torch.from_numpy(Qo).to(cuda), w.M, w.N) == \
+        pytest.approx(O.loglik_gpu(G.to_host_packed(), Po, Qo, w.M), rel=1e-12)

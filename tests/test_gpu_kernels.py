@@ -1,0 +1,148 @@
+"""Parity of every libadx kernel with the CPU oracle, through the C ABI.
+
+Integer/byte work is compared bit-exactly; FP64 EM statistics to 1e-12
+relative (only the summation order differs: MFMA tiles and atomics vs the
+oracle's sequential sums); the FP32 form to 2e-4 relative (FP32 MFMA).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from adamixture_amd import _lib
+from adamixture_amd import genotypes as GT
+from adamixture_amd.em_adam_gpu import EMAdamOptimizer, em_pass
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [  # (M, N, K, missing rate): ragged M and N, K from 1 to 16
+    (37, 29, 3, 0.05),
+    (1001, 777, 7, 0.01),
+    (4101, 130, 16, 0.02),
+    (12, 64, 1, 0.0),
+    (3000, 1000, 10, 0.3),
+]
+
+
+def _params(M, N, K, seed=0):
+    rng = np.random.default_rng(seed)
+    P = np.clip(rng.beta(0.6, 1.2, size=(M, K)), 1e-5, 1 - 1e-5)
+    Q = rng.dirichlet(np.full(K, 0.4), size=N)
+    return P, Q
+
+
+def _device(P, Q, seed, miss, dev):
+    return GT.synthesize(torch.from_numpy(P).to(dev), torch.from_numpy(Q).to(dev), seed, miss)
+
+
+def test_mfma_lane_layout(cuda):
+    rng = np.random.default_rng(1)
+    for kind, dt, tol in ((0, np.float32, 1e-6), (1, np.float64, 1e-14)):
+        A = rng.normal(size=(16, 4)).astype(dt)
+        B = rng.normal(size=(4, 16)).astype(dt)
+        a, b = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda)
+        d = torch.zeros((16, 16), dtype=a.dtype, device=cuda)
+        _lib.call("adx_probe_mfma", kind, a.data_ptr(), b.data_ptr(), d.data_ptr(), _lib.stream_handle(cuda))
+        np.testing.assert_allclose(d.cpu().numpy(), A.astype(np.float64) @ B.astype(np.float64), rtol=tol, atol=tol)
+
+
+@pytest.mark.parametrize("M,N,K,miss", SHAPES)
+def test_synth_bit_exact(cuda, M, N, K, miss):
+    P, Q = _params(M, N, K)
+    G = _device(P, Q, 42, miss, cuda)
+    host = G.data.cpu().numpy()
+    assert np.array_equal(host[:, :N], O.synth_packed(P, Q, 42, miss))
+    assert np.all(host[:, N:] == 0xFF)
+
+
+@pytest.mark.parametrize("M,N,K,miss", SHAPES[:3])
+def test_unpack_freq_qbat_bit_exact(cuda, M, N, K, miss):
+    P, Q = _params(M, N, K, 1)
+    G = _device(P, Q, 7, miss, cuda)
+    Gp = G.to_host_packed()
+    for s, c in ((0, M), (5, 11), (M - 3, 100)):
+        assert np.array_equal(GT.unpack_chunk(G, s, c).cpu().numpy(), O.unpack(Gp, M, s, c))
+    f = torch.rand(M, dtype=torch.float32, device=cuda)
+    got = GT.unpack_center_chunk(G, f, 4, 50).cpu().numpy()
+    assert np.array_equal(got, O.unpack_center(Gp, f.cpu().numpy(), M, 4, 50))
+    assert np.array_equal(GT.allele_frequencies(G).cpu().numpy(), O.freq_gpu(Gp, M))
+    assert np.array_equal(G.qbat().cpu().numpy(), O.qbat(Gp, M))
+    # an unpadded host tensor is repacked into the padded layout bit-exactly
+    G2 = GT.as_packed(torch.from_numpy(Gp), M, cuda)
+    assert G2.ldg % 64 == 0 and np.array_equal(G2.data.cpu().numpy(), G.data.cpu().numpy())
+
+
+def _zeros(dev, dt, *shape):
+    return torch.zeros(*shape, dtype=dt, device=dev)
+
+
+@pytest.mark.parametrize("M,N,K,miss", SHAPES)
+def test_em_pass_f64_matches_oracle(cuda, M, N, K, miss):
+    P, Q = _params(M, N, K, 2)
+    G = _device(P, Q, 11, miss, cuda)
+    Gp = G.to_host_packed()
+    Pd, Qd = torch.from_numpy(P).to(cuda), torch.from_numpy(Q).to(cuda)
+    A, B, T = _zeros(cuda, torch.float64, M, K), _zeros(cuda, torch.float64, M, K), _zeros(cuda, torch.float64, N, K)
+    ll = em_pass(G, Pd, Qd, A, B, T, dtype=_lib.ADX_F64, loglik=True)
+    Ao, Bo, To, qb = O.em_stats_gpu(Gp, P, Q, M)
+    np.testing.assert_allclose(A.cpu().numpy(), Ao, rtol=1e-12, atol=1e-12 * np.abs(Ao).max())
+    np.testing.assert_allclose(B.cpu().numpy(), Bo, rtol=1e-12, atol=1e-12 * np.abs(Bo).max())
+    np.testing.assert_allclose(T.cpu().numpy(), To, rtol=1e-12, atol=1e-12 * np.abs(To).max())
+    assert ll.item() == pytest.approx(O.loglik_gpu(Gp, P, Q, M), rel=1e-12)
+    ll2 = em_pass(G, Pd, Qd, dtype=_lib.ADX_F64, loglik=True, stats=False)
+    assert ll2.item() == pytest.approx(O.loglik_gpu(Gp, P, Q, M), rel=1e-12)
+
+
+@pytest.mark.parametrize("M,N,K,miss", SHAPES)
+def test_em_pass_f32_matches_oracle(cuda, M, N, K, miss):
+    P, Q = _params(M, N, K, 3)
+    G = _device(P, Q, 13, miss, cuda)
+    Gp = G.to_host_packed()
+    P32 = torch.from_numpy(P).to(cuda, torch.float32)
+    Q32 = torch.from_numpy(Q).to(cuda, torch.float32)
+    A, B, T = _zeros(cuda, torch.float32, M, K), _zeros(cuda, torch.float32, M, K), _zeros(cuda, torch.float32, N, K)
+    em_pass(G, P32, Q32, A, B, T, dtype=_lib.ADX_F32)
+    Ao, Bo, To, _ = O.em_stats_gpu(Gp, P32.double().cpu().numpy(), Q32.double().cpu().numpy(), M)
+    for got, want in ((A, Ao), (B, Bo), (T, To)):
+        np.testing.assert_allclose(got.double().cpu().numpy(), want, rtol=2e-4, atol=2e-5 * np.abs(want).max())
+
+
+def test_em_pass_all_missing_rows_and_columns(cuda):
+    M, N, K = 260, 200, 4
+    P, Q = _params(M, N, K, 4)
+    G = _device(P, Q, 5, 0.0, cuda)
+    Gp = G.to_host_packed()
+    Gu = O.unpack(Gp, M)
+    Gu[:, 17] = 3        # a sample with no observed genotype
+    Gu[101] = 3          # a SNP with no observed genotype
+    Gp = O.pack(Gu)
+    G = GT.as_packed(torch.from_numpy(Gp), M, cuda)
+    Pd, Qd = torch.from_numpy(P).to(cuda), torch.from_numpy(Q).to(cuda)
+    opt = EMAdamOptimizer(P.shape, Q.shape, 0.005, 0.8, 0.88, 1e-8, cuda, precision="f64")
+    PE, QE = opt.run_em_step(G, Pd, Qd, M)
+    PEo, QEo = O.em_final_gpu(P, Q, *O.em_stats_gpu(Gp, P, Q, M))
+    np.testing.assert_allclose(PE.cpu().numpy(), PEo, rtol=1e-11, atol=1e-14)
+    np.testing.assert_allclose(QE.cpu().numpy(), QEo, rtol=1e-11, atol=1e-14)
+    assert np.isfinite(PE.cpu().numpy()).all() and np.isfinite(QE.cpu().numpy()).all()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_run_em_step_and_adam_step_match_oracle(cuda, precision):
+    M, N, K = 1500, 700, 6
+    P, Q = _params(M, N, K, 5)
+    G = _device(P, Q, 17, 0.01, cuda)
+    Gp = G.to_host_packed()
+    Pd, Qd = torch.from_numpy(P).to(cuda), torch.from_numpy(Q).to(cuda)
+    opt = EMAdamOptimizer(P.shape, Q.shape, 0.005, 0.8, 0.88, 1e-8, cuda, precision=precision)
+    ref = O.AdamEMGPU(M, N, K, 0.005, 0.8, 0.88, 1e-8)
+    Po, Qo = P.copy(), Q.copy()
+    tol = 1e-11 if precision == "f64" else 1e-4
+    for _ in range(3):
+        PE, QE = opt.run_em_step(G, Pd, Qd, M)
+        PEo, QEo = ref.run_em_step(Gp, Po, Qo, M)
+        np.testing.assert_allclose(PE.cpu().numpy(), PEo, rtol=tol, atol=tol * 1e-3)
+        np.testing.assert_allclose(QE.cpu().numpy(), QEo, rtol=tol, atol=tol * 1e-3)
+        opt.step(Pd, Qd, PE, QE)
+        ref.step(Po, Qo, PE.cpu().numpy(), QE.cpu().numpy())   # same targets: isolates the Adam step
+        np.testing.assert_allclose(Pd.cpu().numpy(), Po, rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(Qd.cpu().numpy(), Qo, rtol=1e-13, atol=1e-15)

@@ -1,0 +1,362 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each test checks an oracle function against known-answer data the reference's
+own tests hold (tests/golden, copied from adamixture/demo) or against a
+property the mathematics fixes, chosen so that a plausible slip (dropped
+term, wrong sign or index, transposed operand) fails.  The planted-mistake
+tests show that the brute-force checks do catch such slips.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import lsq_linear
+
+import oracle as O
+from tests.conftest import GOLDEN
+
+
+def _demo_bed():
+    N = sum(1 for _ in open(GOLDEN / "demo_data.fam"))
+    raw = np.fromfile(GOLDEN / "demo_data.bed", dtype=np.uint8)
+    nb = (N + 3) // 4
+    M = (raw.size - 3) // nb
+    return raw[3:].reshape(M, nb), M, N
+
+
+# ---------------------------------------------------------------- layout ----
+
+def test_reader_fixtures_exact(fixtures):
+    bed, M, N = _demo_bed()
+    G = O.bed_to_unpacked(bed, M, N)
+    Gp = O.bed_to_packed(bed, M, N)
+    assert O.mean_packed(Gp, M) < 0.5          # demo data is not flipped
+    assert np.array_equal(G, fixtures["reader_G"])
+    assert np.array_equal(Gp, fixtures["reader_G_packed"])
+    assert np.array_equal(O.unpack(Gp, M), G)
+    assert np.array_equal(O.pack(G), Gp)
+
+
+def test_package_reader_matches_fixture(fixtures, tmp_path):
+    from adamixture_amd import reader
+    Gp, N, M = reader.read_bed_packed(GOLDEN / "demo_data.bed")
+    assert (N, M) == (105, 8451)
+    assert np.array_equal(Gp.numpy(), fixtures["reader_G_packed"])
+    # write/read round trip of a flipped, ragged matrix (M % 4 = 3, N % 4 = 1)
+    rng = np.random.default_rng(1)
+    G = rng.choice(4, size=(23, 13), p=[0.1, 0.2, 0.6, 0.1]).astype(np.uint8)
+    reader.write_bed(tmp_path / "x", G)
+    Gp2, N2, M2 = reader.read_bed_packed(tmp_path / "x.bed")
+    expect = np.where(G == 3, 3, 2 - G).astype(np.uint8)        # mean >= 0.5 -> flipped
+    assert (N2, M2) == (13, 23)
+    assert np.array_equal(O.unpack(Gp2.numpy(), M2), expect)
+    assert np.all((Gp2.numpy()[-1] >> 6) == 0)                   # padding field zeroed
+
+
+def test_unpack_bruteforce():
+    rng = np.random.default_rng(0)
+    M, N = 37, 29
+    Gp = rng.integers(0, 256, size=((M + 3) // 4, N), dtype=np.uint8)
+    want = np.array([[(Gp[m // 4, i] >> (2 * (m % 4))) & 3 for i in range(N)] for m in range(M)], np.uint8)
+    assert np.array_equal(O.unpack(Gp, M), want)
+    assert np.array_equal(O.unpack(Gp, M, 5, 11), want[5:16])
+    f = rng.random(M).astype(np.float32)
+    c = O.unpack_center(Gp, f, M, 3, 40)
+    ref = np.where(want[3:] == 3, 0.0, want[3:].astype(np.float32) - 2.0 * f[3:, None]).astype(np.float32)
+    assert np.array_equal(c, ref)
+
+
+def test_flip_and_mean_closed_form():
+    rng = np.random.default_rng(2)
+    M, N = 22, 17
+    G = rng.integers(0, 4, size=(M, N)).astype(np.uint8)
+    Gp = O.pack(G)
+    obs = G != 3
+    assert O.mean_packed(Gp, M) == pytest.approx(G[obs].sum() / (2 * obs.sum()), rel=0, abs=1e-15)
+    Gp_pad = Gp.copy()
+    Gp_pad[-1] |= 0b11110000          # garbage in the padding fields (M % 4 = 2)
+    O.flip_packed(Gp_pad, M)
+    assert np.array_equal(O.unpack(Gp_pad, M), np.where(G == 3, 3, 2 - G))
+    assert np.all(Gp_pad[-1] >> 4 == 0)
+
+
+def test_freq_and_qbat_closed_form():
+    rng = np.random.default_rng(3)
+    M, N = 41, 33
+    G = rng.choice(4, size=(M, N), p=[0.3, 0.3, 0.3, 0.1]).astype(np.uint8)
+    G[7] = 3                                   # an all-missing SNP
+    Gp = O.pack(G)
+    obs = G != 3
+    s = np.where(obs, G, 0).sum(1)
+    c = obs.sum(1)
+    want = np.where(c > 0, s / np.maximum(2 * c, 1), 0).astype(np.float32)
+    assert np.array_equal(O.freq_gpu(Gp, M), want)
+    f_cpu = O.freq_cpu(G)
+    assert np.isnan(f_cpu[7]) and np.allclose(np.delete(f_cpu, 7), np.delete(want, 7))
+    assert np.array_equal(O.qbat(Gp, M), 2.0 * obs.sum(0))
+
+
+def test_synth_moments():
+    rng = np.random.default_rng(4)
+    M, N, K = 64, 2000, 3
+    P = rng.uniform(0.05, 0.95, size=(M, K))
+    Q = rng.dirichlet(np.ones(K), size=N)
+    Gp = O.synth_packed(P, Q, seed=42, miss_rate=0.02)
+    G = O.unpack(Gp, M)
+    miss = G == 3
+    assert abs(miss.mean() - 0.02) < 0.004
+    H = P @ Q.T
+    z = (np.where(miss, 0, G) - 2 * H * ~miss).sum() / np.sqrt((2 * H * (1 - H) * ~miss).sum())
+    assert abs(z) < 4.0
+    # a block regenerated on its own is bit-identical
+    assert np.array_equal(O.synth_packed(P, Q, 42, 0.02, row0=5, rows=3), Gp[5:8])
+
+
+# ------------------------------------------------------------ Adam-EM -------
+
+def _tiny(seed=5, M=9, N=7, K=3, miss=0.15):
+    rng = np.random.default_rng(seed)
+    P = rng.uniform(0.02, 0.98, size=(M, K))
+    Q = rng.dirichlet(np.ones(K), size=N)
+    G = rng.choice(4, size=(M, N), p=[(1 - miss) / 3] * 3 + [miss]).astype(np.uint8)
+    return G, P, Q
+
+
+def _brute_stats(G, P, Q, drop_colsum=False):
+    M, K = P.shape
+    N = Q.shape[0]
+    A = np.zeros((M, K)); B = np.zeros((M, K)); T = np.zeros((N, K)); qb = np.zeros(N)
+    for m in range(M):
+        for i in range(N):
+            g = int(G[m, i])
+            if g == 3:
+                continue
+            h = min(max(sum(P[m, k] * Q[i, k] for k in range(K)), 1e-5), 1 - 1e-5)
+            a, b = g / h, (2 - g) / (1 - h)
+            for k in range(K):
+                A[m, k] += a * Q[i, k]
+                B[m, k] += b * Q[i, k]
+                T[i, k] += (a - b) * P[m, k] + (0.0 if drop_colsum else b)
+            qb[i] += 2
+    return A, B, T, qb
+
+
+def test_em_stats_bruteforce():
+    G, P, Q = _tiny()
+    A, B, T, qb = O.em_stats_gpu(O.pack(G), P, Q, G.shape[0])
+    Ab, Bb, Tb, qbb = _brute_stats(G, P, Q)
+    for x, y in ((A, Ab), (B, Bb), (T, Tb), (qb, qbb)):
+        np.testing.assert_allclose(x, y, rtol=1e-13, atol=0)
+
+
+def test_planted_mistake_caught():
+    G, P, Q = _tiny()
+    _, _, T, _ = O.em_stats_gpu(O.pack(G), P, Q, G.shape[0])
+    _, _, Tb, _ = _brute_stats(G, P, Q, drop_colsum=True)
+    assert not np.allclose(T, Tb, rtol=1e-6)
+
+
+def test_em_final_closed_form():
+    G, P, Q = _tiny(K=4)
+    A, B, T, qb = O.em_stats_gpu(O.pack(G), P, Q, G.shape[0])
+    PE, QE = O.em_final_gpu(P, Q, A, B, T, qb)
+    np.testing.assert_allclose(PE, A * P / ((A - B) * P + B), rtol=1e-14)
+    QEb = Q * T / qb[:, None]
+    np.testing.assert_allclose(QE, QEb / QEb.sum(1, keepdims=True), rtol=1e-14)
+    # EM identity: sum_k Q_ik T_ik = q_bat_i when no clamp is active
+    np.testing.assert_allclose((Q * T).sum(1), qb, rtol=1e-12)
+
+
+def test_loglik_bruteforce_and_binomial():
+    from scipy.stats import binom
+    G, P, Q = _tiny(M=11, N=8)
+    Gp = O.pack(G)
+    H = P @ Q.T
+    obs = G != 3
+    want = np.sum(np.where(obs, G * np.log(H) + (2 - G) * np.log1p(-H), 0.0))
+    assert O.loglik_gpu(Gp, P, Q, G.shape[0]) == pytest.approx(want, rel=1e-14)
+    assert O.loglik_cpu(G, P, Q) == pytest.approx(want, rel=1e-14)
+    # equals the binomial log-pmf minus the log binomial coefficients
+    lp = np.sum(np.where(obs, binom.logpmf(np.where(obs, G, 0), 2, H) - np.log(np.where(G == 1, 2.0, 1.0)), 0))
+    assert want == pytest.approx(lp, rel=1e-12)
+
+
+def test_plain_em_ascends():
+    rng = np.random.default_rng(6)
+    M, N, K = 200, 60, 3
+    P = rng.uniform(0.05, 0.95, size=(M, K))
+    Q = rng.dirichlet(np.ones(K), size=N)
+    Gp = O.synth_packed(P, Q, 7, 0.05)
+    P0 = np.clip(rng.random((M, K)), 1e-5, 1 - 1e-5)
+    Q0 = rng.dirichlet(np.ones(K), size=N)
+    last = O.loglik_gpu(Gp, P0, Q0, M)
+    for _ in range(15):
+        A, B, T, qb = O.em_stats_gpu(Gp, P0, Q0, M)
+        P0, Q0 = O.em_final_gpu(P0, Q0, A, B, T, qb)
+        cur = O.loglik_gpu(Gp, P0, Q0, M)
+        assert cur >= last - 1e-9 * abs(last)
+        last = cur
+
+
+def test_adamem_gpu_semantics_matches_reference_fixture(fixtures):
+    """outputs/gpu/adamem: the CUDA FP64 run from its own ALS init."""
+    G, Gp = fixtures["reader_G"], fixtures["reader_G_packed"]
+    M = G.shape[0]
+    P, Q = O.adamem_gpu(Gp, fixtures["gpu_adamem_als_P"], fixtures["gpu_adamem_als_Q"], M)
+    Pe, Qe = fixtures["gpu_adamem_adamem_P"], fixtures["gpu_adamem_adamem_Q"]
+    np.testing.assert_allclose(P, Pe, rtol=2e-2, atol=2e-2)
+    np.testing.assert_allclose(Q, Qe, rtol=2e-2, atol=2e-2)
+    assert abs(O.loglik_cpu(G, P, Q) - O.loglik_cpu(G, Pe, Qe)) <= 50.0
+    assert np.abs(P - Pe).max() < 5e-3        # much tighter than the reference's 2e-2
+
+
+def test_adamem_cpu_semantics_matches_reference_fixture(fixtures):
+    G = fixtures["reader_G"]
+    P, Q = O.adamem_cpu(G, fixtures["cpu_brqn_als_P"], fixtures["cpu_brqn_als_Q"])
+    Pe, Qe = fixtures["cpu_adamem_adamem_P"], fixtures["cpu_adamem_adamem_Q"]
+    np.testing.assert_allclose(P, Pe, rtol=2e-2, atol=2e-2)
+    np.testing.assert_allclose(Q, Qe, rtol=2e-2, atol=2e-2)
+    assert abs(O.loglik_cpu(G, P, Q) - O.loglik_cpu(G, Pe, Qe)) <= 50.0
+
+
+# ------------------------------------------------------------ init ----------
+
+def _centred_rows(G):
+    f = O.freq_cpu(G)
+
+    def rows(s, n):
+        X = G[s:s + n].astype(np.float32) - 2.0 * f[s:s + n, None]
+        X[G[s:s + n] == 3] = 0.0
+        return X
+    return f, rows
+
+
+def _canon(U, V):
+    U, V = U.copy(), V.copy()
+    for c in range(U.shape[1]):
+        p = np.argmax(np.abs(U[:, c]))
+        if U[p, c] < 0:
+            U[:, c] *= -1
+            V[:, c] *= -1
+    return U, V
+
+
+def test_rsvd_matches_fixture_and_dense_svd(fixtures):
+    G = fixtures["reader_G"]
+    M, N = G.shape
+    f, rows = _centred_rows(G)
+    U, S, V = O.rsvd(rows, M, N, 7, 42, 5, 0.1, 8192)
+    U, V = _canon(U, V)
+    np.testing.assert_allclose(S, fixtures["cpu_brqn_svd_S"], rtol=1e-5)
+    np.testing.assert_allclose(U, fixtures["cpu_brqn_svd_U"], atol=1e-5)
+    np.testing.assert_allclose(V, fixtures["cpu_brqn_svd_V"], atol=1e-5)
+    np.testing.assert_allclose(U.T @ U, np.eye(7), atol=1e-4)
+    # with the power iterations run to convergence the result is the dense SVD
+    Sd = np.linalg.svd(rows(0, M).astype(np.float64), compute_uv=False)[:7]
+    _, S2, _ = O.rsvd(rows, M, N, 7, 42, 40, 0.0, 8192)
+    np.testing.assert_allclose(S2, Sd, rtol=1e-4)
+
+
+def test_als_matches_fixture(fixtures):
+    G = fixtures["reader_G"]
+    M, N = G.shape
+    f = O.freq_cpu(G)
+    P, Q = O.als(fixtures["cpu_brqn_svd_U"], fixtures["cpu_brqn_svd_S"], fixtures["cpu_brqn_svd_V"], f, 42,
+                 M, N, 7, 1000, 1e-4)
+    np.testing.assert_allclose(P, fixtures["cpu_brqn_als_P"], atol=1e-6)
+    np.testing.assert_allclose(Q, fixtures["cpu_brqn_als_Q"], atol=1e-6)
+
+
+@pytest.mark.parametrize("variant", ["cpu", "gpu"])
+def test_bvls_kkt_and_lsq_linear(variant):
+    rng = np.random.default_rng(8)
+    K = 6
+    X = rng.normal(size=(30, K))
+    A = X.T @ X + 0.1 * np.eye(K)
+    B = rng.normal(scale=2.0, size=(40, K)) @ A
+    sol = (O.bvls_cpu if variant == "cpu" else O.bvls_gpu)(A, B, 0.0, 1.0)
+    L = np.linalg.cholesky(A)
+    for r in range(B.shape[0]):
+        ref = lsq_linear(L.T, np.linalg.solve(L, B[r]), bounds=(0, 1), method="bvls", tol=1e-12).x
+        np.testing.assert_allclose(sol[r], ref, atol=1e-7)
+        grad = A @ sol[r] - B[r]
+        free = (sol[r] > 1e-9) & (sol[r] < 1 - 1e-9)
+        assert np.all(np.abs(grad[free]) < 1e-7)
+        assert np.all(grad[sol[r] <= 1e-9] >= -1e-7) and np.all(grad[sol[r] >= 1 - 1e-9] <= 1e-7)
+
+
+def test_project_simplex_closed_form():
+    rng = np.random.default_rng(9)
+    for _ in range(50):
+        K = rng.integers(2, 12)
+        b = rng.normal(size=K)
+        got = O.project_simplex(b, 1e-5)
+        # Euclidean projection onto {x >= pc, sum x = 1} by bisection on tau
+        lo, hi = b.min() - 1.0, b.max()
+        for _ in range(200):
+            tau = 0.5 * (lo + hi)
+            s = np.maximum(b - 1e-5 - tau, 0).sum() + K * 1e-5
+            lo, hi = (tau, hi) if s > 1 else (lo, tau)
+        np.testing.assert_allclose(got, np.maximum(b - 1e-5 - tau, 0) + 1e-5, atol=1e-10)
+
+
+def test_sqp_box_qp_is_kkt_optimal():
+    rng = np.random.default_rng(10)
+    K = 5
+    for _ in range(30):
+        X = rng.normal(size=(20, K))
+        H = X.T @ X + 0.5 * np.eye(K)
+        par = rng.uniform(0.05, 0.95, size=K)
+        r = rng.normal(scale=3.0, size=K)
+        tab = np.zeros((K + 1, K + 1))
+        tab[:K, :K] = H
+        tab[:K, K] = r
+        tab[K, :K] = r
+        delta, it = O.qp_sweep(tab, par, K, 0)
+        assert it > 0
+        x = par + delta
+        # maximise r.d - d'Hd/2 s.t. 0 <= par + d <= 1: gradient r - H d
+        grad = r - H @ delta
+        free = (x > 1e-5 + 1e-7) & (x < 1 - 1e-5 - 1e-7)
+        assert np.all(np.abs(grad[free]) < 1e-6)
+        assert np.all(x >= -1e-9) and np.all(x <= 1 + 1e-9)
+
+
+def test_grad_hess_finite_differences():
+    rng = np.random.default_rng(11)
+    M, N, K = 15, 9, 3
+    P = rng.uniform(0.1, 0.9, size=(M, K))
+    Q = rng.dirichlet(np.ones(K), size=N)
+    Gp = O.synth_packed(P, Q, 3, 0.1)
+    Hq, zq = O.grad_hess_q(Gp, Q, P, M)
+    Hp, zp = O.grad_hess_p(Gp, Q, P, M)
+    eps = 1e-6
+    i, k = 4, 1
+    Qp, Qm = Q.copy(), Q.copy()
+    Qp[i, k] += eps
+    Qm[i, k] -= eps
+    d = (O.loglik_gpu(Gp, P, Qp, M) - O.loglik_gpu(Gp, P, Qm, M)) / (2 * eps)
+    # z_q = sum_j a p_j + b (1 - p_j) = dlogL/dQ_ik + sum_j b_j
+    Gu = O.unpack(Gp, M)[:, i].astype(float)
+    obs = Gu != 3
+    h = P @ Q[i]
+    b = np.where(obs, (2 - Gu) / (1 - h), 0.0)
+    assert zq[i, k] - b.sum() == pytest.approx(d, rel=1e-5)
+    j = 6
+    Pp, Pm = P.copy(), P.copy()
+    Pp[j, k] += eps
+    Pm[j, k] -= eps
+    d = (O.loglik_gpu(Gp, Pp, Q, M) - O.loglik_gpu(Gp, Pm, Q, M)) / (2 * eps)
+    assert zp[j, k] == pytest.approx(d, rel=1e-5)
+    # Hessian: minus the second derivative of logL along P_jk
+    e2 = 1e-4
+    Pp[j, k] = P[j, k] + e2
+    Pm[j, k] = P[j, k] - e2
+    d2 = (O.loglik_gpu(Gp, Pp, Q, M) - 2 * O.loglik_gpu(Gp, P, Q, M) + O.loglik_gpu(Gp, Pm, Q, M)) / e2**2
+    assert Hp[j, k, k] == pytest.approx(-d2, rel=1e-5)
+
+
+def test_brqn_matches_reference_fixture(fixtures):
+    Gp = fixtures["reader_G_packed"]
+    M = fixtures["reader_G"].shape[0]
+    P, Q = O.brqn(Gp, fixtures["cpu_brqn_als_P"], fixtures["cpu_brqn_als_Q"], M)
+    np.testing.assert_allclose(P, fixtures["cpu_brqn_brqn_P"], atol=1e-8)
+    np.testing.assert_allclose(Q, fixtures["cpu_brqn_brqn_Q"], atol=1e-8)
