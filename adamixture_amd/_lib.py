@@ -42,6 +42,8 @@ SIGNATURES = {
     "adx_em_finalize": (_int, [_int, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _p, _p]),
     "adx_adam_step": (_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _dbl, _dbl, _dbl, _dbl,
                              _dbl, _p, _p, _i64, _p]),
+    "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
+    "adx_bvls_batch": (_int, [_p, _p, _p, _p, _i64, _int, _dbl, _dbl, _p, _i64, _p]),
     "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),
 }
 

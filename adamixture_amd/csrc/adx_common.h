@@ -33,10 +33,12 @@ int fail_msg(int code, const std::string& msg);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// One 16x16x4 MFMA step; identical lane layouts for FP32 and FP64:
+// One 16x16x4 MFMA step.  Operand layouts are shared by FP32 and FP64:
 //   A (16x4):  lane l supplies A[l % 16][l / 16]
 //   B (4x16):  lane l supplies B[l / 16][l % 16]
-//   D (16x16): lane l holds D[4 * (l / 16) + j][l % 16], j = 0..3
+// The accumulator layouts differ; with g = l / 16 and j = 0..3 lane l holds
+//   FP32: D[4 g + j][l % 16]        FP64: D[g + 4 j][l % 16]
+// row(g, j) gives that row index (pinned by the adx_probe_mfma test).
 template <typename T>
 struct Mma16;
 
@@ -46,6 +48,7 @@ struct Mma16<float> {
     __device__ static inline acc run(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __host__ __device__ static constexpr int row(int g, int j) { return 4 * g + j; }
 };
 
 template <>
@@ -54,6 +57,17 @@ struct Mma16<double> {
     __device__ static inline acc run(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __host__ __device__ static constexpr int row(int g, int j) { return g + 4 * j; }
 };
+
+// Inverse of row(): the lane group g and element j holding row r.
+template <typename T>
+__host__ __device__ constexpr int row_group(int r) {
+    return Mma16<T>::row(1, 0) == 4 ? r >> 2 : r & 3;
+}
+template <typename T>
+__host__ __device__ constexpr int row_elem(int r) {
+    return Mma16<T>::row(1, 0) == 4 ? r & 3 : r >> 2;
+}
 
 }  // namespace adx

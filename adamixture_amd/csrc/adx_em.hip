@@ -141,13 +141,15 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
             }
         }
 
-        // Row blocks of 16 samples: sample offset of D-row 4g'+j in row block
-        // rw is 16g'+4rw+j, so lane (g, cl) reads bytes 4rw..4rw+3 of its 16.
+        // Row blocks of 16 samples: the accumulator element j of lane group g
+        // (D-row MM::row(g, j)) is sample s0+16g+4rw+j, so lane (g, cl) reads
+        // bytes 4rw..4rw+3 of its 16.
 #pragma unroll
         for (int rw = 0; rw < 4; ++rw) {
             T qH[KS];
             T qT[4];
-            const int64_t sH = s0 + 16 * (cl >> 2) + 4 * rw + (cl & 3);
+            // D-row r holds the sample of lane group row_group(r), element row_elem(r).
+            const int64_t sH = s0 + 16 * row_group<T>(cl) + 4 * rw + row_elem<T>(cl);
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const int kp = 4 * ks + g;
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                     for (int j = 0; j < 4; ++j) {
                         accA[cb] = MM::run(qT[j], a[j], accA[cb]);
                         accB[cb] = MM::run(qT[j], b[j], accB[cb]);
-                        s_tile[wave][cl * 16 + 4 * g + j] = a[j] - b[j];
+                        s_tile[wave][cl * 16 + MM::row(g, j)] = a[j] - b[j];
                         sb[j] += b[j];
                     }
 #pragma unroll
@@ -223,7 +225,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
             if (snp < M) {
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) {
-                    const int kp = 4 * g + jj;
+                    const int kp = MM::row(g, jj);
                     if (kp < K) {
                         atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
                         atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);

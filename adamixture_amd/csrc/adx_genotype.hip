@@ -188,14 +188,20 @@ __global__ void repack_kernel(const uint8_t* __restrict__ src, int64_t ld_src, i
 // Lane-layout probe: one wave runs one 16x16x4 MFMA on A (16x4) and B (4x16)
 // given row-major, using the operand mapping documented in adx_common.h,
 // and writes D back row-major with the documented accumulator mapping.
+// raw != 0 writes D[l * 4 + j] = accumulator element j of lane l instead.
 template <typename T>
-__global__ void probe_mma_kernel(const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ D) {
+__global__ void probe_mma_kernel(const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ D, int raw) {
     const int l = threadIdx.x;
     using acc_t = typename Mma16<T>::acc;
     acc_t d = {0, 0, 0, 0};
     d = Mma16<T>::run(A[(l % 16) * 4 + l / 16], B[(l / 16) * 16 + l % 16], d);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) D[(4 * (l / 16) + j) * 16 + (l % 16)] = d[j];
+    for (int j = 0; j < 4; ++j) {
+        if (raw)
+            D[l * 4 + j] = d[j];
+        else
+            D[Mma16<T>::row(l / 16, j) * 16 + (l % 16)] = d[j];
+    }
 }
 
 }  // namespace adx
@@ -273,12 +279,14 @@ int adx_repack_padded(const uint8_t* src, int64_t ld_src, int64_t Mb, int64_t N,
 }
 
 int adx_probe_mfma(int kind, const void* A, const void* B, void* D, void* stream) {
-    if (kind == 0)
-        probe_mma_kernel<float><<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D);
-    else if (kind == 1)
-        probe_mma_kernel<double><<<1, 64, 0, (hipStream_t)stream>>>((const double*)A, (const double*)B, (double*)D);
+    const int raw = kind >= 2;
+    if (kind == 0 || kind == 2)
+        probe_mma_kernel<float><<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D, raw);
+    else if (kind == 1 || kind == 3)
+        probe_mma_kernel<double><<<1, 64, 0, (hipStream_t)stream>>>((const double*)A, (const double*)B, (double*)D,
+                                                                     raw);
     else
-        return fail_msg(1, "adx_probe_mfma: kind must be 0 (f32) or 1 (f64)");
+        return fail_msg(1, "adx_probe_mfma: kind must be 0..3");
     ADX_LAUNCH_CHECK("adx_probe_mfma");
     return 0;
 }

@@ -107,11 +107,35 @@ int adx_adam_step(double *P, double *Q, const double *P_EM, const double *Q_EM, 
                   double *mQ, double *vQ, int64_t M, int64_t N, int K, double t, double lr, double beta1,
                   double beta2, double eps, float *P_lo, float *Q_lo, int64_t ld_lo, void *stream);
 
+/* --- batched K x K solvers (FP64, 1 <= K <= 64) ---------------------------- */
+
+/* Replaces torch.ops.sqp_kernel.sqp_solve_q_cuda (simplex = 1) and
+ * sqp_solve_p_cuda (simplex = 0) (adamixture/src/utils_c/cuda/
+ * sqp_kernel.cu:262-370): for each of nprob problems, the sweep-operator
+ * active-set QP on the tableau built from H (nprob x K x K), r (nprob x K)
+ * and the current point X (nprob x K) -- r enters negated, as the reference
+ * op expects the caller's negated gradient (br_qn_gpu.py:295-300) -- then
+ * Xn = X + delta projected on the simplex with pseudocount 1e-5 (Q rows, v_kk
+ * of svd(ones(1, K)) sets the mu shift, br_qn_gpu.py:248-250) or clipped to
+ * [1e-5, 1-1e-5] (P rows).  workspace: >= 8 bytes (problem queue). */
+int adx_sqp_solve(int simplex, const double *H, const double *r, const double *X, const double *vkk,
+                  double *Xn, int64_t nprob, int K, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Replaces torch.ops.bvls_kernel.batch_bvls_bpp_cuda (adamixture/src/utils_c/
+ * cuda/bvls_kernel.cu:188-218): for each row b of B (nprob x K), the bounded
+ * least squares min 1/2 x'Ax - b'x, lo <= x <= hi by block principal pivoting.
+ * With Ainv = inv(A) the pivoting starts from X0 = inv(A) b as the CUDA op
+ * does; with Ainv = NULL it starts from the Gauss-Jordan solution as the
+ * Cython batch_bvls_bpp (bvls.pyx:130-226).  workspace: >= 8 bytes. */
+int adx_bvls_batch(const double *A, const double *Ainv, const double *B, double *X, int64_t nprob, int K,
+                   double lo, double hi, void *workspace, int64_t workspace_bytes, void *stream);
+
 /* --- tests ----------------------------------------------------------------- */
 
 /* No reference counterpart: runs one 16x16x4 MFMA (kind 0 = FP32, 1 = FP64)
  * on row-major A (16x4) and B (4x16) with the lane mapping the EM kernels
- * assume, writing D (16x16) row-major.  Used to pin that mapping. */
+ * assume, writing D (16x16) row-major.  kind 2 / 3 (FP32 / FP64) write the
+ * raw accumulators instead, D[lane * 4 + j].  Used to pin the mapping. */
 int adx_probe_mfma(int kind, const void *A, const void *B, void *D, void *stream);
 
 #ifdef __cplusplus

@@ -35,15 +35,30 @@ def _device(P, Q, seed, miss, dev):
     return GT.synthesize(torch.from_numpy(P).to(dev), torch.from_numpy(Q).to(dev), seed, miss)
 
 
+# accumulator row held by lane group g, element j (adx_common.h)
+ROW = {0: lambda g, j: 4 * g + j, 1: lambda g, j: g + 4 * j}
+
+
 def test_mfma_lane_layout(cuda):
     rng = np.random.default_rng(1)
     for kind, dt, tol in ((0, np.float32, 1e-6), (1, np.float64, 1e-14)):
         A = rng.normal(size=(16, 4)).astype(dt)
         B = rng.normal(size=(4, 16)).astype(dt)
+        want = A.astype(np.float64) @ B.astype(np.float64)
         a, b = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda)
+        raw = torch.zeros(256, dtype=a.dtype, device=cuda)
+        _lib.call("adx_probe_mfma", kind + 2, a.data_ptr(), b.data_ptr(), raw.data_ptr(), _lib.stream_handle(cuda))
+        raw = raw.cpu().numpy()
+        # which (row, col) of A @ B each raw accumulator holds
+        where = [np.unravel_index(np.argmin(np.abs(want - raw[l * 4 + j])), want.shape)
+                 for l in range(64) for j in range(4)]
+        print(f"kind {kind}: lane 16 holds rows {[where[16 * 4 + j][0] for j in range(4)]}")
+        for l in range(64):
+            for j in range(4):
+                assert where[l * 4 + j] == (ROW[kind](l // 16, j), l % 16), (kind, l, j, where[l * 4 + j])
         d = torch.zeros((16, 16), dtype=a.dtype, device=cuda)
         _lib.call("adx_probe_mfma", kind, a.data_ptr(), b.data_ptr(), d.data_ptr(), _lib.stream_handle(cuda))
-        np.testing.assert_allclose(d.cpu().numpy(), A.astype(np.float64) @ B.astype(np.float64), rtol=tol, atol=tol)
+        np.testing.assert_allclose(d.cpu().numpy(), want, rtol=tol, atol=tol)
 
 
 @pytest.mark.parametrize("M,N,K,miss", SHAPES)
