@@ -42,6 +42,9 @@ SIGNATURES = {
     "adx_em_finalize": (_int, [_int, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _p, _p]),
     "adx_adam_step": (_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _dbl, _dbl, _dbl, _dbl,
                              _dbl, _p, _p, _i64, _p]),
+    "adx_em_bf16_operand_bytes": (_i64, [_i64, _i64, _int, _p]),
+    "adx_em_prep_bf16": (_int, [_p, _p, _i64, _i64, _int, _p, _i64, _p]),
+    "adx_em_pass_bf16": (_int, [_p, _i64, _i64, _i64, _int, _p, _i64, _p, _p, _p, _p]),
     "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
     "adx_bvls_batch": (_int, [_p, _p, _p, _p, _i64, _int, _dbl, _dbl, _p, _i64, _p]),
     "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),

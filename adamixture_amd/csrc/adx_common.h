@@ -71,3 +71,17 @@ __host__ __device__ constexpr int row_elem(int r) {
 }
 
 }  // namespace adx
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+namespace adx {
+
+// 16x16x32 BF16 MFMA with FP32 accumulation (gfx950):
+//   A (16x32): lane l supplies A[l % 16][8 (l / 16) + i], i = 0..7
+//   B (32x16): lane l supplies B[8 (l / 16) + i][l % 16]
+//   D (16x16): lane l holds D[4 (l / 16) + j][l % 16]   (as FP32 16x16x4)
+__device__ inline floatx4 mma_bf16(bf16x8 a, bf16x8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+}  // namespace adx

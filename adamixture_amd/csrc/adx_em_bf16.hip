@@ -1,0 +1,335 @@
+// BF16 form of the fused decode + EM-statistics pass (FP32 accumulation).
+//
+// Same tiling as em_pass_kernel (adx_em.hip): a workgroup owns 64*CW SNPs and
+// a sample strip, lane (g, c) streams 16 genotype bytes of one byte row per
+// 64-sample sub-tile, A and B live in MFMA accumulators for the strip and T
+// is reduced per sub-tile.  The contractions run on 16x16x32 BF16 MFMA:
+//
+//   H^T  = Qx . Px^T with Qx = [Q_hi, Q_hi, Q_lo], Px = [P_hi, P_lo, P_hi]
+//          (three-term split: |H - H_exact| <~ 3 * 2^-16 |H|); one MFMA for
+//          K <= 10, two for K <= 16
+//   A^T += (Qt_hi + Qt_lo) . a^T,  B^T += (Qt_hi + Qt_lo) . b^T
+//          k = 32 samples (two 16-sample row blocks), a, b rounded to BF16
+//   T   += [a | b]^T . [P_hi ; 1 - P_hi]      k = 16 SNPs x {a, b}; the
+//          [a | b] tile goes through a 1 KB per-wave LDS transpose and the
+//          colsum(b) term of em_batch_math is the (1 - P) half.
+//
+// a = g/H and b = (2-g)/(1-H) are formed in FP32 from the FP32 accumulator,
+// with one reciprocal of H(1-H).  The operand arrays Px, Qx, Qt are rebuilt
+// from the FP64 P, Q by adx_em_prep_bf16 before every pass.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "adx_common.h"
+#include "../../include/adx.h"
+
+namespace adx {
+
+__device__ __forceinline__ void em_terms_f32(float h, uint32_t code, float& a, float& b) {
+    const float hc = fminf(fmaxf(h, 1e-5f), 1.0f - 1e-5f);
+    const float u = 1.0f - hc;
+    const float r = __builtin_amdgcn_rcpf(hc * u);
+    const float ca = code == 3u ? 0.0f : (float)code;
+    const float cb = code == 3u ? 0.0f : 2.0f - (float)code;
+    a = ca * (u * r);
+    b = cb * (hc * r);
+}
+
+__device__ __forceinline__ bf16x8 load_bf16x8(const __bf16* p) {
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+
+template <int KH, int CW>
+__global__ __launch_bounds__(256) void em_pass_bf16_kernel(
+    const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
+    const __bf16* __restrict__ Qx, const __bf16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
+    float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc) {
+    static_assert(CW % 4 == 0, "CW must be a multiple of 4");
+    constexpr int LX = 32 * KH;  // row length of Px / Qx
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
+    const int64_t snp_w = ((int64_t)blockIdx.x * 4 + wave) * (16 * CW);
+    const int64_t row_w = snp_w >> 2;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
+    const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
+
+    __shared__ __attribute__((aligned(16))) __bf16 s_tile[4][2][16][16];   // per wave: [a|b][row][SNP col]
+    __shared__ float s_tsum[64 * 16];
+
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) s_tsum[e] = 0.0f;
+
+    const bf16x8 zero8 = bf16x8{};
+    bf16x8 pH[CW][KH];
+    bf16x8 pT[CW];
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        const int t = cb >> 2, f = cb & 3;
+        const int64_t snp = snp_w + 64 * t + 4 * cl + f;
+#pragma unroll
+        for (int kh = 0; kh < KH; ++kh) pH[cb][kh] = snp < M ? load_bf16x8(Px + snp * LX + 32 * kh + 8 * g) : zero8;
+        // T B-operand: k = 8g + i -> (type g >> 1, SNP column 8 (g & 1) + i), column kp = cl.
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t s2 = snp_w + 64 * t + 4 * (8 * (g & 1) + i) + f;
+            float x = 0.0f;
+            if (s2 < M && cl < K) {
+                const float p = (float)Px[s2 * LX + cl];
+                x = (g >> 1) ? 1.0f - p : p;
+            }
+            v[i] = (__bf16)x;
+        }
+        pT[cb] = v;
+    }
+
+    floatx4 accA[CW], accB[CW];
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        accA[cb] = floatx4{0, 0, 0, 0};
+        accB[cb] = floatx4{0, 0, 0, 0};
+    }
+    __syncthreads();
+
+    for (int64_t s0 = n0; s0 < n1; s0 += 64) {
+        uint32_t gw[CW / 4][4];
+#pragma unroll
+        for (int t = 0; t < CW / 4; ++t) {
+            const int64_t row = row_w + 16 * t + cl;
+            if (row < Mb) {
+                const uint4 v = *reinterpret_cast<const uint4*>(G + row * ldg + s0 + 16 * g);
+                gw[t][0] = v.x;
+                gw[t][1] = v.y;
+                gw[t][2] = v.z;
+                gw[t][3] = v.w;
+                const int64_t nf = M - 4 * row;
+                if (nf < 4) {
+                    const uint32_t pm = ((0xFFu << (2 * nf)) & 0xFFu) * 0x01010101u;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) gw[t][r] |= pm;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) gw[t][r] = 0xFFFFFFFFu;
+            }
+        }
+
+        // Row-block pairs: accumulator element j of lane group g in row block
+        // rw is sample s0 + 16g + 4rw + j; a pair (rw = 2tp, 2tp+1) gives lane
+        // (g, cl) the 8 consecutive samples s0 + 16g + 8tp + i, i = 0..7.
+#pragma unroll
+        for (int tp = 0; tp < 2; ++tp) {
+            const int64_t sq = s0 + 16 * g + 8 * tp;  // first of the 8 samples (Qt rows are padded)
+            const bf16x8 qThi = load_bf16x8(Qt + (int64_t)cl * ldqt + sq);
+            const bf16x8 qTlo = load_bf16x8(Qt + (16 + (int64_t)cl) * ldqt + sq);
+            bf16x8 qH[2][KH];
+#pragma unroll
+            for (int rr = 0; rr < 2; ++rr) {
+                const int64_t sH = s0 + 16 * (cl >> 2) + 4 * (2 * tp + rr) + (cl & 3);
+#pragma unroll
+                for (int kh = 0; kh < KH; ++kh)
+                    qH[rr][kh] = sH < N ? load_bf16x8(Qx + sH * LX + 32 * kh + 8 * g) : zero8;
+            }
+            floatx4 accT[2] = {floatx4{0, 0, 0, 0}, floatx4{0, 0, 0, 0}};
+
+#pragma unroll
+            for (int cb = 0; cb < CW; ++cb) {
+                const int t = cb >> 2, f = cb & 3;
+                bf16x8 aB, bB;
+#pragma unroll
+                for (int rr = 0; rr < 2; ++rr) {
+                    floatx4 h = floatx4{0, 0, 0, 0};
+#pragma unroll
+                    for (int kh = 0; kh < KH; ++kh) h = mma_bf16(qH[rr][kh], pH[cb][kh], h);
+                    const uint32_t w = gw[t][2 * tp + rr] >> (2 * f);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float a, b;
+                        em_terms_f32(h[j], (w >> (8 * j)) & 3u, a, b);
+                        const __bf16 ab = (__bf16)a, bb = (__bf16)b;
+                        aB[4 * rr + j] = ab;
+                        bB[4 * rr + j] = bb;
+                        s_tile[wave][0][4 * g + j][cl] = ab;
+                        s_tile[wave][1][4 * g + j][cl] = bb;
+                    }
+                    // T (16 samples x 16 kp) += [a|b] (16 x 32) . [P ; 1-P] (32 x 16)
+                    const bf16x8 tv = *reinterpret_cast<const bf16x8*>(&s_tile[wave][g >> 1][cl][8 * (g & 1)]);
+                    accT[rr] = mma_bf16(tv, pT[cb], accT[rr]);
+                }
+                accA[cb] = mma_bf16(qThi, aB, accA[cb]);
+                accA[cb] = mma_bf16(qTlo, aB, accA[cb]);
+                accB[cb] = mma_bf16(qThi, bB, accB[cb]);
+                accB[cb] = mma_bf16(qTlo, bB, accB[cb]);
+            }
+#pragma unroll
+            for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (cl < K) atomicAdd(&s_tsum[(16 * g + 4 * (2 * tp + rr) + j) * 16 + cl], accT[rr][j]);
+        }
+
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+            const int kp = e & 15;
+            const int64_t smp = s0 + (e >> 4);
+            if (kp < K && smp < N) atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+            s_tsum[e] = 0.0f;
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        const int t = cb >> 2, f = cb & 3;
+        const int64_t snp = snp_w + 64 * t + 4 * cl + f;
+        if (snp < M) {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int kp = 4 * g + jj;
+                if (kp < K) {
+                    atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
+                    atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
+                }
+            }
+        }
+    }
+}
+
+// Operand arrays of the BF16 pass from the FP64 master P (M x K) and Q (N x K):
+//   Px[m]  = [P_hi(0..K), P_lo, P_hi, 0 ...]    (32*KH bf16)
+//   Qx[i]  = [Q_hi, Q_hi, Q_lo, 0 ...]          (32*KH bf16)
+//   Qt     = [hi rows 0..15 | lo rows 0..15] x ldqt, Qt[kp][i] = Q[i][kp]
+// with x_hi = bf16(x), x_lo = bf16(x - x_hi); zero outside the valid range.
+template <int KH>
+__global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, __bf16* __restrict__ Px) {
+    constexpr int LX = 32 * KH;
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    __bf16 row[LX];
+#pragma unroll
+    for (int e = 0; e < LX; ++e) row[e] = (__bf16)0.0f;
+    for (int k = 0; k < K; ++k) {
+        const float x = (float)P[m * K + k];
+        const __bf16 hi = (__bf16)x;
+        const __bf16 lo = (__bf16)(x - (float)hi);
+        row[k] = hi;
+        row[K + k] = lo;
+        row[2 * K + k] = hi;
+    }
+#pragma unroll
+    for (int e = 0; e < LX; ++e) Px[m * LX + e] = row[e];
+}
+
+template <int KH>
+__global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, __bf16* __restrict__ Qx,
+                               __bf16* __restrict__ Qt, int64_t ldqt) {
+    constexpr int LX = 32 * KH;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ldqt) return;
+    __bf16 row[LX];
+#pragma unroll
+    for (int e = 0; e < LX; ++e) row[e] = (__bf16)0.0f;
+    for (int k = 0; k < 16; ++k) {
+        __bf16 hi = (__bf16)0.0f, lo = (__bf16)0.0f;
+        if (i < N && k < K) {
+            const float x = (float)Q[i * K + k];
+            hi = (__bf16)x;
+            lo = (__bf16)(x - (float)hi);
+            row[k] = hi;
+            row[K + k] = hi;
+            row[2 * K + k] = lo;
+        }
+        Qt[(int64_t)k * ldqt + i] = hi;
+        Qt[(int64_t)(16 + k) * ldqt + i] = lo;
+    }
+    if (i < N)
+#pragma unroll
+        for (int e = 0; e < LX; ++e) Qx[i * LX + e] = row[e];
+}
+
+struct Bf16Grid {
+    int64_t nbx, splits, n_per_split;
+};
+
+constexpr int BF16_CW = 8;
+
+static Bf16Grid bf16_grid(int64_t M, int64_t N) {
+    Bf16Grid gd;
+    gd.nbx = ceil_div(M, 64 * BF16_CW);
+    int64_t s = ceil_div(2048, gd.nbx);
+    const int64_t max_s = ceil_div(N, 64);
+    if (s > max_s) s = max_s;
+    if (s < 1) s = 1;
+    if (s > 65535) s = 65535;
+    gd.n_per_split = ceil_div(ceil_div(N, s), 64) * 64;
+    gd.splits = ceil_div(N, gd.n_per_split);
+    return gd;
+}
+
+}  // namespace adx
+
+using namespace adx;
+
+extern "C" {
+
+int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t* ldqt) {
+    const int kh = K <= 10 ? 1 : 2;
+    const int64_t ld = ceil_div(N, 64) * 64;
+    if (ldqt) *ldqt = ld;
+    return (int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 32 * ld);
+}
+
+int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int K, void* operands,
+                     int64_t operand_bytes, void* stream) {
+    if (K < 1 || K > 16) return fail_msg(1, "adx_em_prep_bf16: K must be in [1, 16]");
+    int64_t ldqt = 0;
+    if (operand_bytes < adx_em_bf16_operand_bytes(M, N, K, &ldqt))
+        return fail_msg(1, "adx_em_prep_bf16: operand buffer too small");
+    const int kh = K <= 10 ? 1 : 2;
+    __bf16* Px = static_cast<__bf16*>(operands);
+    __bf16* Qx = Px + M * 32 * kh;
+    __bf16* Qt = Qx + N * 32 * kh;
+    hipStream_t st = (hipStream_t)stream;
+    if (kh == 1) {
+        if (M > 0) prep_px_kernel<1><<<(unsigned)ceil_div(M, 256), 256, 0, st>>>(P, M, K, Px);
+        prep_qx_kernel<1><<<(unsigned)ceil_div(ldqt, 256), 256, 0, st>>>(Q, N, K, Qx, Qt, ldqt);
+    } else {
+        if (M > 0) prep_px_kernel<2><<<(unsigned)ceil_div(M, 256), 256, 0, st>>>(P, M, K, Px);
+        prep_qx_kernel<2><<<(unsigned)ceil_div(ldqt, 256), 256, 0, st>>>(Q, N, K, Qx, Qt, ldqt);
+    }
+    ADX_LAUNCH_CHECK("adx_em_prep_bf16");
+    return 0;
+}
+
+int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* operands,
+                     int64_t operand_bytes, float* A, float* B, float* T, void* stream) {
+    if (M <= 0 || N <= 0) return fail_msg(1, "adx_em_pass_bf16: empty genotype matrix");
+    if (K < 1 || K > 16) return fail_msg(1, "adx_em_pass_bf16: K must be in [1, 16]");
+    if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64)
+        return fail_msg(1, "adx_em_pass_bf16: ldg must be a multiple of 16 and >= N rounded up to 64");
+    if (reinterpret_cast<uintptr_t>(G) % 16 != 0 || reinterpret_cast<uintptr_t>(operands) % 16 != 0)
+        return fail_msg(1, "adx_em_pass_bf16: G and operands must be 16-byte aligned");
+    int64_t ldqt = 0;
+    if (operand_bytes < adx_em_bf16_operand_bytes(M, N, K, &ldqt))
+        return fail_msg(1, "adx_em_pass_bf16: operand buffer too small");
+    const int kh = K <= 10 ? 1 : 2;
+    const __bf16* Px = static_cast<const __bf16*>(operands);
+    const __bf16* Qx = Px + M * 32 * kh;
+    const __bf16* Qt = Qx + N * 32 * kh;
+    const Bf16Grid gd = bf16_grid(M, N);
+    dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
+    hipStream_t st = (hipStream_t)stream;
+    if (kh == 1)
+        em_pass_bf16_kernel<1, BF16_CW><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A,
+                                                              B, T);
+    else
+        em_pass_bf16_kernel<2, BF16_CW><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A,
+                                                              B, T);
+    ADX_LAUNCH_CHECK("adx_em_pass_bf16");
+    return 0;
+}
+
+}  // extern "C"

@@ -204,6 +204,21 @@ __global__ void probe_mma_kernel(const T* __restrict__ A, const T* __restrict__ 
     }
 }
 
+// BF16 probe: A (16x32) and B (32x16) given row-major as FP32 (values exact
+// in bf16); raw accumulators written to D[lane * 4 + j].
+__global__ void probe_bf16_kernel(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ D) {
+    const int l = threadIdx.x;
+    bf16x8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a[i] = (__bf16)A[(l % 16) * 32 + 8 * (l / 16) + i];
+        b[i] = (__bf16)B[(8 * (l / 16) + i) * 16 + l % 16];
+    }
+    floatx4 d = mma_bf16(a, b, floatx4{0, 0, 0, 0});
+#pragma unroll
+    for (int j = 0; j < 4; ++j) D[l * 4 + j] = d[j];
+}
+
 }  // namespace adx
 
 using namespace adx;
@@ -280,13 +295,18 @@ int adx_repack_padded(const uint8_t* src, int64_t ld_src, int64_t Mb, int64_t N,
 
 int adx_probe_mfma(int kind, const void* A, const void* B, void* D, void* stream) {
     const int raw = kind >= 2;
+    if (kind == 4) {
+        probe_bf16_kernel<<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D);
+        ADX_LAUNCH_CHECK("adx_probe_mfma");
+        return 0;
+    }
     if (kind == 0 || kind == 2)
         probe_mma_kernel<float><<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D, raw);
     else if (kind == 1 || kind == 3)
         probe_mma_kernel<double><<<1, 64, 0, (hipStream_t)stream>>>((const double*)A, (const double*)B, (double*)D,
                                                                      raw);
     else
-        return fail_msg(1, "adx_probe_mfma: kind must be 0..3");
+        return fail_msg(1, "adx_probe_mfma: kind must be 0..4");
     ADX_LAUNCH_CHECK("adx_probe_mfma");
     return 0;
 }

@@ -10,6 +10,8 @@ Precision forms (``precision``):
     float64 on CUDA); H and the A/B/T contractions on FP64 MFMA.
   * ``"f32"``: H and the contractions on FP32 MFMA from FP32 copies of P and
     Q; P, Q, the Adam state and the log-likelihood stay FP64.
+  * ``"bf16"``: H (three-term hi/lo split) and the contractions on BF16 MFMA
+    with FP32 accumulation; P, Q, Adam state and log-likelihood FP64.
 """
 from __future__ import annotations
 
@@ -24,7 +26,7 @@ from .genotypes import PackedGenotypes, as_packed
 
 log = logging.getLogger(__name__)
 
-PRECISIONS = ("f64", "f32")
+PRECISIONS = ("f64", "f32", "bf16")
 
 
 def default_precision() -> str:
@@ -90,20 +92,33 @@ class EMAdamOptimizer:
         N = Q_shape[0]
         z = lambda *s, dt=torch.float64: torch.zeros(*s, dtype=dt, device=self.device)  # noqa: E731
         self.m_P, self.v_P, self.m_Q, self.v_Q = z(M, K), z(M, K), z(N, K), z(N, K)
-        acc_dt = torch.float32 if self.precision == "f32" else torch.float64
-        self.acc_dtype = _lib.ADX_F32 if self.precision == "f32" else _lib.ADX_F64
+        low = self.precision in ("f32", "bf16")
+        acc_dt = torch.float32 if low else torch.float64
+        self.acc_dtype = _lib.ADX_F32 if low else _lib.ADX_F64
         self.A_accum, self.B_accum = z(M, K, dt=acc_dt), z(M, K, dt=acc_dt)
         self.T_accum = z(N, K, dt=acc_dt)
         self.P_EM, self.Q_EM = z(M, K), z(N, K)
         if self.precision == "f32":
             self.P_lo = torch.zeros(M, K, dtype=torch.float32, device=self.device)
             self.Q_lo = torch.zeros(N, K, dtype=torch.float32, device=self.device)
+        if self.precision == "bf16":
+            nbytes = _lib.load().adx_em_bf16_operand_bytes(M, N, K, None)
+            self.operands = torch.zeros(nbytes, dtype=torch.uint8, device=self.device)
 
     def accumulate(self, G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, loglik: bool = False):
         """EM statistics at (P, Q) into A/B/T; optionally the FP64 logL at the
         same point (fused into the same pass in the f64 form)."""
         if self.precision == "f64":
             return em_pass(G, P, Q, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F64, loglik=loglik)
+        if self.precision == "bf16":
+            st = _lib.stream_handle(self.device)
+            M, K = P.shape
+            N = Q.shape[0]
+            nb = self.operands.numel()
+            _lib.call("adx_em_prep_bf16", P.data_ptr(), Q.data_ptr(), M, N, K, self.operands.data_ptr(), nb, st)
+            _lib.call("adx_em_pass_bf16", G.data.data_ptr(), G.ldg, G.M, G.N, K, self.operands.data_ptr(), nb,
+                      self.A_accum.data_ptr(), self.B_accum.data_ptr(), self.T_accum.data_ptr(), st)
+            return em_pass(G, P, Q, dtype=_lib.ADX_F64, loglik=True, stats=False) if loglik else None
         self.P_lo.copy_(P)
         self.Q_lo.copy_(Q)
         em_pass(G, self.P_lo, self.Q_lo, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F32)

@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="biobank")
-    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "f32", "f64"])
     ap.add_argument("--cpu-rows", type=int, default=None,
                     help="SNPs in the bounded CPU-baseline sample (default: ~10-30 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -98,6 +98,17 @@ def main():
             ll = opt.accumulate(G, P, Q, loglik=need_ll)
             if timed:
                 e1.record()
+        elif args.precision == "bf16":
+            st = _lib.stream_handle(dev)
+            nb = opt.operands.numel()
+            _lib.call("adx_em_prep_bf16", P.data_ptr(), Q.data_ptr(), w.M, w.N, w.K, opt.operands.data_ptr(), nb, st)
+            if timed:
+                e0.record()
+            _lib.call("adx_em_pass_bf16", G.data.data_ptr(), G.ldg, G.M, G.N, w.K, opt.operands.data_ptr(), nb,
+                      opt.A_accum.data_ptr(), opt.B_accum.data_ptr(), opt.T_accum.data_ptr(), st)
+            if timed:
+                e1.record()
+            ll = em_pass(G, P, Q, dtype=_lib.ADX_F64, loglik=True, stats=False) if need_ll else None
         else:
             opt.P_lo.copy_(P)
             opt.Q_lo.copy_(Q)
@@ -124,7 +135,7 @@ def main():
     elapsed = time.perf_counter() - t0
     kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
-    s = 8 if args.precision == "f64" else 4
+    s = 8 if args.precision == "f64" else 4      # bytes per P/Q/A/B/T element (FP32 accumulators)
     M, N, K = w.M, w.N, w.K
     flops = 8.0 * K * N * M
     bytes_alg = ((M + 3) // 4) * N + (M * K + N * K) * s + (2 * M * K + N * K) * s
@@ -153,7 +164,8 @@ def main():
                    "precision": args.precision, "logl_check_every": check, "init": "random (seeded)"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "kernel": "em_pass_kernel", "kernel_ms": round(kern_ms, 4),
+                     "kernel": "em_pass_bf16_kernel" if args.precision == "bf16" else "em_pass_kernel",
+                     "kernel_ms": round(kern_ms, 4),
                      "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg,
                      "hbm_GBs_algorithmic": round(bytes_alg / (kern_ms * 1e-3) / 1e9, 1)},
     }

@@ -90,6 +90,18 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
                 const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
                 int flags, double *ll_out, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* BF16 form of adx_em_pass (ADX_EM_STATS only, 1 <= K <= 16), FP32 A, B, T.
+ * Same contract as adx_em_pass with dtype ADX_F32, but H and the A/B/T
+ * contractions run on BF16 MFMA with FP32 accumulation from operand arrays
+ * that adx_em_prep_bf16 builds from the FP64 P and Q (H from a three-term
+ * hi/lo split; A and B from Q split in hi + lo; a, b rounded to BF16).
+ * Replaces the same reference code (em_adam_gpu.py:46-78, :166-197). */
+int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t *ldqt);
+int adx_em_prep_bf16(const double *P, const double *Q, int64_t M, int64_t N, int K, void *operands,
+                     int64_t operand_bytes, void *stream);
+int adx_em_pass_bf16(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, const void *operands,
+                     int64_t operand_bytes, float *A, float *B, float *T, void *stream);
+
 /* Replaces em_final_update (adamixture/model/em_adam_gpu.py:82-117):
  * P_EM = A P / max((A-B) P + B, 1e-5); Q_EM = Q T / max(qbat, 1e-5), rows
  * divided by max(row sum, 1e-5).  A, B, T have dtype acc_dtype (F32/F64) and
@@ -135,7 +147,9 @@ int adx_bvls_batch(const double *A, const double *Ainv, const double *B, double 
 /* No reference counterpart: runs one 16x16x4 MFMA (kind 0 = FP32, 1 = FP64)
  * on row-major A (16x4) and B (4x16) with the lane mapping the EM kernels
  * assume, writing D (16x16) row-major.  kind 2 / 3 (FP32 / FP64) write the
- * raw accumulators instead, D[lane * 4 + j].  Used to pin the mapping. */
+ * raw accumulators instead, D[lane * 4 + j]; kind 4 runs one 16x16x32 BF16
+ * MFMA on FP32 A (16x32) and B (32x16) holding bf16-exact values, writing raw
+ * accumulators.  Used to pin the lane mappings. */
 int adx_probe_mfma(int kind, const void *A, const void *B, void *D, void *stream);
 
 #ifdef __cplusplus

@@ -64,7 +64,7 @@ def test_demo_adamem_f64_matches_oracle_and_reference_fixture(cuda, fixtures):
     assert abs(O.loglik_cpu(G, P, Q) - O.loglik_cpu(G, Pe, Qe)) <= 50.0
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
 def test_cpu_ref_workload_fixed_iterations(cuda, precision):
     """N=1,000 x M=20,000, K=3, 1% missing: 100 Adam-EM iterations from the
     same init on the GPU and on the oracle (FP64)."""

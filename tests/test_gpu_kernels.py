@@ -61,6 +61,23 @@ def test_mfma_lane_layout(cuda):
         np.testing.assert_allclose(d.cpu().numpy(), want, rtol=tol, atol=tol)
 
 
+def test_mfma_bf16_lane_layout(cuda):
+    """16x16x32 BF16 MFMA: operands A[l%16][8(l/16)+i], B[8(l/16)+i][l%16],
+    accumulator element j of lane l = D[4(l/16)+j][l%16]."""
+    rng = np.random.default_rng(2)
+    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)      # exact in bf16
+    B = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
+    A[np.arange(16), np.arange(16)] += 50.0 * (1 + np.arange(16))   # make rows of A @ B distinct
+    want = A.astype(np.float64) @ B.astype(np.float64)
+    raw = torch.zeros(256, dtype=torch.float32, device=cuda)
+    a, b = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda)
+    _lib.call("adx_probe_mfma", 4, a.data_ptr(), b.data_ptr(), raw.data_ptr(), _lib.stream_handle(cuda))
+    raw = raw.cpu().numpy().astype(np.float64)
+    for l in range(64):
+        for j in range(4):
+            assert raw[l * 4 + j] == want[4 * (l // 16) + j, l % 16], (l, j)
+
+
 @pytest.mark.parametrize("M,N,K,miss", SHAPES)
 def test_synth_bit_exact(cuda, M, N, K, miss):
     P, Q = _params(M, N, K)
@@ -161,3 +178,24 @@ def test_run_em_step_and_adam_step_match_oracle(cuda, precision):
         ref.step(Po, Qo, PE.cpu().numpy(), QE.cpu().numpy())   # same targets: isolates the Adam step
         np.testing.assert_allclose(Pd.cpu().numpy(), Po, rtol=1e-13, atol=1e-15)
         np.testing.assert_allclose(Qd.cpu().numpy(), Qo, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("M,N,K,miss", SHAPES)
+def test_em_pass_bf16_error_vs_oracle(cuda, M, N, K, miss):
+    """BF16 form against the FP64 oracle at the same (P, Q).  Stated bound:
+    2e-3 relative on A, B, T (H from the three-term split is good to ~5e-5;
+    the rest is the rounding of a, b, Q and P to BF16, averaged over the
+    contraction)."""
+    P, Q = _params(M, N, K, 3)
+    G = _device(P, Q, 13, miss, cuda)
+    Gp = G.to_host_packed()
+    Pd, Qd = torch.from_numpy(P).to(cuda), torch.from_numpy(Q).to(cuda)
+    opt = EMAdamOptimizer(P.shape, Q.shape, 0.005, 0.8, 0.88, 1e-8, cuda, precision="bf16")
+    opt.accumulate(G, Pd, Qd)
+    Ao, Bo, To, _ = O.em_stats_gpu(Gp, P, Q, M)
+    errs = []
+    for got, want in ((opt.A_accum, Ao), (opt.B_accum, Bo), (opt.T_accum, To)):
+        g64 = got.double().cpu().numpy()
+        errs.append(np.abs(g64 - want).max() / np.abs(want).max())
+        np.testing.assert_allclose(g64, want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
+    print(f"bf16 M={M} N={N} K={K}: max rel err A {errs[0]:.2e} B {errs[1]:.2e} T {errs[2]:.2e}")
