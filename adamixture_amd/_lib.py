@@ -45,6 +45,9 @@ SIGNATURES = {
     "adx_em_bf16_operand_bytes": (_i64, [_i64, _i64, _int, _p]),
     "adx_em_prep_bf16": (_int, [_p, _p, _i64, _i64, _int, _p, _i64, _p]),
     "adx_em_pass_bf16": (_int, [_p, _i64, _i64, _i64, _int, _p, _i64, _p, _p, _p, _p]),
+    "adx_svd_xw": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p]),
+    "adx_svd_xty": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p]),
+    "adx_grad_hess": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _p, _p, _p, _p]),
     "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
     "adx_bvls_batch": (_int, [_p, _p, _p, _p, _i64, _int, _dbl, _dbl, _p, _i64, _p]),
     "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),

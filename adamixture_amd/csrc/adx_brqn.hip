@@ -1,0 +1,231 @@
+// Gradient and Hessian blocks of the SQP block relaxation (BR-QN), FP64.
+//
+// Reference: adamixture/model/br_qn_gpu.py:67-214 (_grad_hess_q_chunk,
+// _grad_hess_p_chunk, compute_grad_hess_{Q,P}_gpu) and the Cython twin
+// sqp.pyx:258-328.  With H = P Q^T clamped to [1e-10, 1-1e-10] and, on
+// observed genotypes, z1 = g/H, z2 = (2-g)/(1-H), w1 = z1/H, w2 = z2/(1-H):
+//   Q side (per sample i):  Xtz_q = sum_j z1 p_j + z2 (1-p_j)
+//                           XtX_q = sum_j w1 p_j p_j^T + w2 (1-p_j)(1-p_j)^T
+//   P side (per SNP j):     Xtz_p = sum_i (z1 - z2) q_i
+//                           XtX_p = sum_i (w1 + w2) q_i q_i^T
+// The reference forms these with einsum over C x N x K x K temporaries.  Here
+// each thread owns one problem's K(K+1)/2 + K accumulators in registers and
+// streams the 2-bit genotypes once:
+//   Q side: thread = sample, workgroup = 256 consecutive samples x a SNP
+//           range; one coalesced byte per 4 SNPs, P rows are wave-uniform.
+//   P side: thread = SNP, workgroup = 64 SNPs; 256-sample chunks of the
+//           genotype rows and of Q are staged in LDS (Q reads broadcast).
+// Sample splits / SNP splits are combined with FP64 atomics.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "adx_common.h"
+#include "../../include/adx.h"
+
+namespace adx {
+
+struct Weights {
+    double z1, z2, w1, w2;
+};
+
+__device__ __forceinline__ Weights bq_weights(double h, uint32_t code) {
+    Weights w{0.0, 0.0, 0.0, 0.0};
+    if (code == 3u) return w;
+    const double hc = fmax(fmin(h, 1.0 - 1e-10), 1e-10);
+    const double g = (double)code;
+    w.z1 = g / hc;
+    w.z2 = (2.0 - g) / (1.0 - hc);
+    w.w1 = w.z1 / hc;
+    w.w2 = w.z2 / (1.0 - hc);
+    return w;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                          int64_t N, int K, const double* __restrict__ Q,
+                                                          const double* __restrict__ P, int64_t rows_per_split,
+                                                          double* __restrict__ XtX, double* __restrict__ Xtz) {
+    constexpr int KT = KM * (KM + 1) / 2;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
+    const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
+    const bool live = i < N;
+    double q[KM], hz[KM], hx[KT];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        q[k] = (live && k < K) ? Q[i * K + k] : 0.0;
+        hz[k] = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < KT; ++t) hx[t] = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+        const uint32_t byte = live ? G[r * ldg + i] : 0xFFu;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            const int64_t m = 4 * r + f;
+            if (m >= M) break;
+            const uint32_t code = (byte >> (2 * f)) & 3u;
+            double p[KM];
+            double h = 0.0;
+#pragma unroll
+            for (int k = 0; k < KM; ++k) {
+                p[k] = k < K ? P[m * K + k] : 0.0;
+                h += q[k] * p[k];
+            }
+            const Weights w = bq_weights(h, code);
+            double a[KM], b[KM];
+#pragma unroll
+            for (int k = 0; k < KM; ++k) {
+                const double u = 1.0 - p[k];
+                hz[k] += w.z1 * p[k] + w.z2 * u;
+                a[k] = w.w1 * p[k];
+                b[k] = w.w2 * u;
+            }
+            int t = 0;
+#pragma unroll
+            for (int k = 0; k < KM; ++k)
+#pragma unroll
+                for (int l = k; l < KM; ++l, ++t) hx[t] += a[k] * p[l] + b[k] * (1.0 - p[l]);
+        }
+    }
+    if (!live) return;
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        if (k < K) atomicAdd(&Xtz[i * K + k], hz[k]);
+#pragma unroll
+        for (int l = k; l < KM; ++l, ++t)
+            if (k < K && l < K) {
+                atomicAdd(&XtX[(i * K + k) * K + l], hx[t]);
+                if (l != k) atomicAdd(&XtX[(i * K + l) * K + k], hx[t]);
+            }
+    }
+}
+
+template <int KM>
+__global__ __launch_bounds__(64) void grad_hess_p_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                         int64_t N, int K, const double* __restrict__ Q,
+                                                         const double* __restrict__ P, int64_t n_per_split,
+                                                         double* __restrict__ XtX, double* __restrict__ Xtz) {
+    constexpr int KT = KM * (KM + 1) / 2;
+    constexpr int CH = 256;  // samples per LDS chunk
+    __shared__ uint8_t sg[16][CH];
+    __shared__ double sq[CH][KM];
+    const int tid = threadIdx.x;
+    const int64_t m = (int64_t)blockIdx.x * 64 + tid;
+    const int64_t row0 = (int64_t)blockIdx.x * 16;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
+    const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
+    const bool live = m < M;
+    double p[KM], hz[KM], hx[KT];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        p[k] = (live && k < K) ? P[m * K + k] : 0.0;
+        hz[k] = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < KT; ++t) hx[t] = 0.0;
+    for (int64_t c0 = n0; c0 < n1; c0 += CH) {
+        const int nc = (int)(n1 - c0 < CH ? n1 - c0 : CH);
+        __syncthreads();
+        for (int e = tid; e < 16 * CH; e += 64) {
+            const int r = e / CH, c = e - r * CH;
+            sg[r][c] = (row0 + r < Mb && c < nc) ? G[(row0 + r) * ldg + c0 + c] : 0xFFu;
+        }
+        for (int e = tid; e < CH * KM; e += 64) {
+            const int c = e / KM, k = e - c * KM;
+            sq[c][k] = (c < nc && k < K) ? Q[(c0 + c) * K + k] : 0.0;
+        }
+        __syncthreads();
+        if (!live) continue;
+        const int rr = tid >> 2, sh = 2 * (tid & 3);
+        for (int c = 0; c < nc; ++c) {
+            const uint32_t code = (sg[rr][c] >> sh) & 3u;
+            double h = 0.0;
+#pragma unroll
+            for (int k = 0; k < KM; ++k) h += sq[c][k] * p[k];
+            const Weights w = bq_weights(h, code);
+            const double zd = w.z1 - w.z2, ws = w.w1 + w.w2;
+            double wq[KM];
+#pragma unroll
+            for (int k = 0; k < KM; ++k) {
+                const double qk = sq[c][k];
+                hz[k] += zd * qk;
+                wq[k] = ws * qk;
+            }
+            int t = 0;
+#pragma unroll
+            for (int k = 0; k < KM; ++k)
+#pragma unroll
+                for (int l = k; l < KM; ++l, ++t) hx[t] += wq[k] * sq[c][l];
+        }
+    }
+    if (!live) return;
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        if (k < K) atomicAdd(&Xtz[m * K + k], hz[k]);
+#pragma unroll
+        for (int l = k; l < KM; ++l, ++t)
+            if (k < K && l < K) {
+                atomicAdd(&XtX[(m * K + k) * K + l], hx[t]);
+                if (l != k) atomicAdd(&XtX[(m * K + l) * K + k], hx[t]);
+            }
+    }
+}
+
+template <int KM>
+static void launch_gh(int side, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
+                      const double* Q, const double* P, double* XtX, double* Xtz) {
+    const int64_t Mb = ceil_div(M, 4);
+    if (side == 0) {
+        const int64_t nbx = ceil_div(N, 256);
+        int64_t s = ceil_div(4096, nbx);
+        s = s > Mb ? Mb : (s < 1 ? 1 : s);
+        if (s > 65535) s = 65535;
+        const int64_t rps = ceil_div(Mb, s);
+        dim3 grid((unsigned)nbx, (unsigned)ceil_div(Mb, rps));
+        grad_hess_q_kernel<KM><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, rps, XtX, Xtz);
+    } else {
+        const int64_t nbx = ceil_div(M, 64);
+        int64_t s = ceil_div(8192, nbx);
+        const int64_t max_s = ceil_div(N, 256);
+        s = s > max_s ? max_s : (s < 1 ? 1 : s);
+        if (s > 65535) s = 65535;
+        const int64_t nps = ceil_div(ceil_div(N, s), 256) * 256;
+        dim3 grid((unsigned)nbx, (unsigned)ceil_div(N, nps));
+        grad_hess_p_kernel<KM><<<grid, 64, 0, st>>>(G, ldg, M, N, K, Q, P, nps, XtX, Xtz);
+    }
+}
+
+}  // namespace adx
+
+using namespace adx;
+
+extern "C" {
+
+int adx_grad_hess(int side, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const double* Q,
+                  const double* P, double* XtX, double* Xtz, void* stream) {
+    if (M <= 0 || N <= 0) return 0;
+    if (K < 1 || K > 16) return fail_msg(1, "adx_grad_hess: K must be in [1, 16]");
+    if (side != 0 && side != 1) return fail_msg(1, "adx_grad_hess: side must be 0 (Q) or 1 (P)");
+    if (ldg < N) return fail_msg(1, "adx_grad_hess: ldg < N");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t rows = side == 0 ? N : M;
+    ADX_CHECK(hipMemsetAsync(XtX, 0, sizeof(double) * (size_t)(rows * K * K), st));
+    ADX_CHECK(hipMemsetAsync(Xtz, 0, sizeof(double) * (size_t)(rows * K), st));
+    if (K <= 4)
+        launch_gh<4>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
+    else if (K <= 8)
+        launch_gh<8>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
+    else if (K <= 12)
+        launch_gh<12>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
+    else
+        launch_gh<16>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
+    ADX_LAUNCH_CHECK("adx_grad_hess");
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+// Decode-fused GEMMs of the randomized-SVD initializer (FP32, as the
+// reference's SGEMMs).  X is the centred genotype matrix of
+// pack2bit.cu:46-77 / tools.pyx:292-324: X[m, i] = g - 2 f[m], 0 if missing.
+// The reference materialises X chunk by chunk (C x N FP32) and calls cuBLAS
+// (svd_gpu.py:86, :106-107, :138); here the 2-bit codes are decoded inside the
+// GEMM and X never exists in memory.
+//
+//   svd_xw_kernel:   Y = X W          (M x L; W is N x L)    reduction over samples
+//   svd_xty_kernel:  A += X^T Y       (N x L; Y is M x L)    reduction over SNPs
+//
+// Both run 16x16x4 FP32 MFMA tiles with L padded to 16*LT rows:
+//   xw:  Y^T (L x SNPs) += W^T (L x 4 samples) . X^T (4 samples x SNPs).  The
+//        genotype operand comes straight from the EM kernel's per-lane 16-byte
+//        row loads (lane (g, c): byte row 16t+c, samples s0+16g .. s0+16g+15).
+//   xty: A^T (L x samples) += Y^T (L x 4 SNPs) . X (4 SNPs x samples).  A
+//        64-row x 64-sample genotype tile is staged in LDS so that lane (g, c)
+//        reads field g of byte [row s][sample c] for the k-step of byte row s.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "adx_common.h"
+#include "../../include/adx.h"
+
+namespace adx {
+
+__device__ __forceinline__ float centred(uint32_t code, float twof) {
+    return code == 3u ? 0.0f : (float)code - twof;
+}
+
+template <int LT, int CW>
+__global__ __launch_bounds__(256) void svd_xw_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                     int64_t N, const float* __restrict__ f,
+                                                     const float* __restrict__ W, int L, int64_t n_per_split,
+                                                     float* __restrict__ Y) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
+    const int64_t snp_w = ((int64_t)blockIdx.x * 4 + wave) * (16 * CW);
+    const int64_t row_w = snp_w >> 2;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
+    const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
+
+    float twof[CW];
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        const int64_t snp = snp_w + 64 * (cb >> 2) + 4 * cl + (cb & 3);
+        twof[cb] = snp < M ? 2.0f * f[snp] : 0.0f;
+    }
+    floatx4 acc[CW][LT];
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb)
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) acc[cb][lt] = floatx4{0, 0, 0, 0};
+
+    for (int64_t s0 = n0; s0 < n1; s0 += 64) {
+        uint32_t gw[CW / 4][4];
+#pragma unroll
+        for (int t = 0; t < CW / 4; ++t) {
+            const int64_t row = row_w + 16 * t + cl;
+            if (row < Mb) {
+                const uint4 v = *reinterpret_cast<const uint4*>(G + row * ldg + s0 + 16 * g);
+                gw[t][0] = v.x; gw[t][1] = v.y; gw[t][2] = v.z; gw[t][3] = v.w;
+                const int64_t nf = M - 4 * row;
+                if (nf < 4) {
+                    const uint32_t pm = ((0xFFu << (2 * nf)) & 0xFFu) * 0x01010101u;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) gw[t][r] |= pm;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) gw[t][r] = 0xFFFFFFFFu;
+            }
+        }
+        // k-step e: the 4 samples s0 + 16g + e (g = 0..3).
+#pragma unroll 4
+        for (int e = 0; e < 16; ++e) {
+            const int64_t smp = s0 + 16 * g + e;
+            float wv[LT];
+#pragma unroll
+            for (int lt = 0; lt < LT; ++lt) {
+                const int li = 16 * lt + cl;
+                wv[lt] = (smp < N && li < L) ? W[smp * L + li] : 0.0f;
+            }
+#pragma unroll
+            for (int cb = 0; cb < CW; ++cb) {
+                const uint32_t code = (gw[cb >> 2][e >> 2] >> (8 * (e & 3) + 2 * (cb & 3))) & 3u;
+                const float x = centred(code, twof[cb]);
+#pragma unroll
+                for (int lt = 0; lt < LT; ++lt) acc[cb][lt] = Mma16<float>::run(wv[lt], x, acc[cb][lt]);
+            }
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < CW; ++cb) {
+        const int64_t snp = snp_w + 64 * (cb >> 2) + 4 * cl + (cb & 3);
+        if (snp >= M) continue;
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int li = 16 * lt + 4 * g + j;
+                if (li < L) atomicAdd(&Y[snp * L + li], acc[cb][lt][j]);
+            }
+    }
+}
+
+// Workgroup: 64 samples (16 per wave), one SNP range; byte-row chunks of 64
+// rows staged in LDS.
+template <int LT>
+__global__ __launch_bounds__(256) void svd_xty_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                      int64_t N, const float* __restrict__ f,
+                                                      const float* __restrict__ Y, int L, int64_t rows_per_split,
+                                                      float* __restrict__ A) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
+    const int64_t s0 = (int64_t)blockIdx.x * 64;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
+    const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
+    __shared__ __attribute__((aligned(16))) uint8_t tile[64][64];
+
+    floatx4 acc[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = floatx4{0, 0, 0, 0};
+    const int64_t smp = s0 + 16 * wave + cl;   // this lane's sample (B-operand column)
+
+    for (int64_t rb = r0; rb < r1; rb += 64) {
+        {   // stage 64 byte rows x 64 samples: thread -> row tid/4, 16 bytes at 16*(tid%4)
+            const int r = threadIdx.x >> 2, c = 16 * (threadIdx.x & 3);
+            uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+            if (rb + r < r1) v = *reinterpret_cast<const uint4*>(G + (rb + r) * ldg + s0 + c);
+            *reinterpret_cast<uint4*>(&tile[r][c]) = v;
+        }
+        __syncthreads();
+        const int nrows = (int)(r1 - rb < 64 ? r1 - rb : 64);
+        for (int s = 0; s < nrows; ++s) {
+            const int64_t snp = 4 * (rb + s) + g;
+            uint32_t code = (tile[s][16 * wave + cl] >> (2 * g)) & 3u;
+            float twof = 0.0f;
+            if (snp < M) twof = 2.0f * f[snp];
+            else code = 3u;
+            const float x = centred(code, twof);
+#pragma unroll
+            for (int lt = 0; lt < LT; ++lt) {
+                const int li = 16 * lt + cl;
+                const float yv = (snp < M && li < L) ? Y[snp * L + li] : 0.0f;
+                acc[lt] = Mma16<float>::run(yv, x, acc[lt]);
+            }
+        }
+        __syncthreads();
+    }
+    if (smp < N) {
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int li = 16 * lt + 4 * g + j;
+                if (li < L) atomicAdd(&A[smp * L + li], acc[lt][j]);
+            }
+    }
+}
+
+template <int LT>
+static void launch_xw(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f,
+                      const float* W, int L, float* Y) {
+    constexpr int CW = 4;
+    const int64_t nbx = ceil_div(M, 64 * CW);
+    int64_t s = ceil_div(2048, nbx);
+    const int64_t max_s = ceil_div(N, 64);
+    s = s > max_s ? max_s : (s < 1 ? 1 : s);
+    if (s > 65535) s = 65535;
+    const int64_t nps = ceil_div(ceil_div(N, s), 64) * 64;
+    dim3 grid((unsigned)nbx, (unsigned)ceil_div(N, nps));
+    svd_xw_kernel<LT, CW><<<grid, 256, 0, st>>>(G, ldg, M, N, f, W, L, nps, Y);
+}
+
+template <int LT>
+static void launch_xty(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f,
+                       const float* Y, int L, float* A) {
+    const int64_t Mb = ceil_div(M, 4);
+    const int64_t nbx = ceil_div(N, 64);
+    int64_t s = ceil_div(2048, nbx);
+    const int64_t max_s = ceil_div(Mb, 64);
+    s = s > max_s ? max_s : (s < 1 ? 1 : s);
+    if (s > 65535) s = 65535;
+    const int64_t rps = ceil_div(ceil_div(Mb, s), 64) * 64;
+    dim3 grid((unsigned)nbx, (unsigned)ceil_div(Mb, rps));
+    svd_xty_kernel<LT><<<grid, 256, 0, st>>>(G, ldg, M, N, f, Y, L, rps, A);
+}
+
+}  // namespace adx
+
+using namespace adx;
+
+extern "C" {
+
+int adx_svd_xw(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f, const float* W, int L,
+               float* Y, void* stream) {
+    if (M <= 0 || N <= 0) return 0;
+    if (L < 1 || L > 64) return fail_msg(1, "adx_svd_xw: L must be in [1, 64]");
+    if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64) return fail_msg(1, "adx_svd_xw: bad ldg");
+    hipStream_t st = (hipStream_t)stream;
+    switch ((L + 15) / 16) {
+        case 1: launch_xw<1>(st, G, ldg, M, N, f, W, L, Y); break;
+        case 2: launch_xw<2>(st, G, ldg, M, N, f, W, L, Y); break;
+        case 3: launch_xw<3>(st, G, ldg, M, N, f, W, L, Y); break;
+        default: launch_xw<4>(st, G, ldg, M, N, f, W, L, Y); break;
+    }
+    ADX_LAUNCH_CHECK("adx_svd_xw");
+    return 0;
+}
+
+int adx_svd_xty(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f, const float* Y, int L,
+                float* A, void* stream) {
+    if (M <= 0 || N <= 0) return 0;
+    if (L < 1 || L > 64) return fail_msg(1, "adx_svd_xty: L must be in [1, 64]");
+    if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64) return fail_msg(1, "adx_svd_xty: bad ldg");
+    hipStream_t st = (hipStream_t)stream;
+    switch ((L + 15) / 16) {
+        case 1: launch_xty<1>(st, G, ldg, M, N, f, Y, L, A); break;
+        case 2: launch_xty<2>(st, G, ldg, M, N, f, Y, L, A); break;
+        case 3: launch_xty<3>(st, G, ldg, M, N, f, Y, L, A); break;
+        default: launch_xty<4>(st, G, ldg, M, N, f, Y, L, A); break;
+    }
+    ADX_LAUNCH_CHECK("adx_svd_xty");
+    return 0;
+}
+
+}  // extern "C"

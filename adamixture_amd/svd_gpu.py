@@ -1,0 +1,106 @@
+"""Randomized SVD initializer on the GPU (drop-in for adamixture/model/svd_gpu.py).
+
+The streamed GEMMs with the centred genotype matrix run in libadx
+(adx_svd_xw / adx_svd_xty, decode fused, FP32 MFMA); the L x L
+eigen-decompositions (L = max(k + 10, 20)) run on the host in FP32 LAPACK,
+as the reference CPU path does (svd.py:13-28).
+
+The Gaussian test matrix is drawn on the host with
+``numpy.random.default_rng(seed).standard_normal((M, L), float32)`` exactly
+as svd.py:52-62 does, so the GPU result reproduces the CPU reference's
+numbers (the reference GPU path draws from torch's device generator,
+svd_gpu.py:68-71, which no other device reproduces).
+"""
+from __future__ import annotations
+
+import logging
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .genotypes import PackedGenotypes, as_packed
+
+log = logging.getLogger(__name__)
+
+
+def eigSVD_host(C: np.ndarray):
+    """svd.py:13-28 eigSVD of C (n x L): eigh of C^T C, descending order."""
+    D, V = np.linalg.eigh(C.T @ C)
+    S = np.sqrt(D)
+    U = C @ (V * (1.0 / S))
+    return (np.ascontiguousarray(U[:, ::-1]), np.ascontiguousarray(S[::-1]), np.ascontiguousarray(V[:, ::-1]))
+
+
+def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """Y = X W (M x L) with X the centred genotypes, W (N x L) FP32."""
+    L = W.shape[1]
+    Y = torch.zeros((G.M, L), dtype=torch.float32, device=G.device)
+    _lib.call("adx_svd_xw", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.contiguous().data_ptr(), L,
+              Y.data_ptr(), _lib.stream_handle(G.device))
+    return Y
+
+
+def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+    """A = X^T Y (N x L) with Y (M x L) FP32."""
+    L = Y.shape[1]
+    A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
+    _lib.call("adx_svd_xty", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.contiguous().data_ptr(), L,
+              A.data_ptr(), _lib.stream_handle(G.device))
+    return A
+
+
+def _SVD_gpu_once(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: int, tol: float,
+                  chunk_size: int, device, threads_per_block: int = 0):
+    """svd.py:30-140 / svd_gpu.py:40-154 (dashSVD with dynamic shift)."""
+    t0 = time.time()
+    dev = torch.device(device)
+    G = as_packed(G, M, dev)
+    f = f.to(device=dev, dtype=torch.float32).contiguous()
+    L = max(k + 10, 20)
+    rng = np.random.default_rng(seed)
+    omega = torch.from_numpy(rng.standard_normal(size=(M, L), dtype=np.float32)).to(dev)
+
+    accum = xty(G, f, omega).cpu().numpy()
+    orth, _, _ = eigSVD_host(accum)
+    alpha = 0.0
+    sv = np.zeros(L, dtype=np.float32)
+    for i in range(power):
+        proj = xw(G, f, torch.from_numpy(orth).to(dev))
+        accum = xty(G, f, proj).cpu().numpy()
+        accum -= alpha * orth
+        orth, s_vals, _ = eigSVD_host(accum)
+        if i > 0:
+            s_cur = s_vals + alpha
+            rel = np.abs(s_cur - sv[:len(s_cur)]) / np.maximum(s_cur, 1e-12)
+            if np.max(rel[:k]) < tol:
+                log.info(f"        Converged at iteration {i}.")
+                break
+            sv[:len(s_cur)] = s_cur
+        else:
+            sv[:len(s_vals)] = s_vals + alpha
+        if s_vals[-1] > alpha:
+            alpha = 0.5 * (s_vals[-1] + alpha)
+    proj = xw(G, f, torch.from_numpy(orth).to(dev)).cpu().numpy()
+    u, s_all, v = eigSVD_host(proj)
+    U = torch.from_numpy(np.ascontiguousarray(u[:, :k])).to(dev)
+    S = torch.from_numpy(np.ascontiguousarray(s_all[:k])).to(dev)
+    V = torch.from_numpy(np.ascontiguousarray(orth @ v[:, :k])).to(dev)
+    log.info(f"\n    Total time for SVD={time.time() - t0:.3f}s")
+    return U, S, V
+
+
+def SVD_gpu(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: int, tol: float, chunk_size: int,
+            device, threads_per_block: int = 0):
+    """svd_gpu.py:157-178 with the OOM retry (no chunk temporaries here, so a
+    retry can only come from the O(M L) buffers)."""
+    from .utils import MIN_GPU_CHUNK_SIZE, is_gpu_oom, reduce_gpu_chunk_or_raise
+    cur = chunk_size
+    while True:
+        try:
+            return _SVD_gpu_once(G, N, M, f, k, seed, power, tol, cur, device, threads_per_block)
+        except RuntimeError as exc:
+            if not is_gpu_oom(exc) or cur <= MIN_GPU_CHUNK_SIZE:
+                raise
+            cur = reduce_gpu_chunk_or_raise(cur, exc, "running SVD")

@@ -119,6 +119,33 @@ int adx_adam_step(double *P, double *Q, const double *P_EM, const double *Q_EM, 
                   double *mQ, double *vQ, int64_t M, int64_t N, int K, double t, double lr, double beta1,
                   double beta2, double eps, float *P_lo, float *Q_lo, int64_t ld_lo, void *stream);
 
+/* --- randomized-SVD initializer GEMMs (FP32) -------------------------------- */
+
+/* X is the centred genotype matrix of pack2bit.cu:46-77 (x = g - 2 f[m], 0 if
+ * missing), decoded inside the GEMM.  Replace the chunked unpack + SGEMM of
+ * adamixture/model/svd_gpu.py:104-107 and :133-138:
+ *   adx_svd_xw:  Y (M x L, row-major) += X W, W is N x L
+ *   adx_svd_xty: A (N x L, row-major) += X^T Y, Y is M x L
+ * Outputs are accumulated (atomically): zero them first.  1 <= L <= 64. */
+int adx_svd_xw(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float *f, const float *W, int L,
+               float *Y, void *stream);
+int adx_svd_xty(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float *f, const float *Y, int L,
+                float *A, void *stream);
+
+/* --- SQP block-relaxation gradient / Hessian (FP64, 1 <= K <= 16) ---------- */
+
+/* Replaces compute_grad_hess_Q_gpu (side 0) and compute_grad_hess_P_gpu
+ * (side 1) with their chunk math (adamixture/model/br_qn_gpu.py:67-214):
+ * with H = P Q^T clamped to [1e-10, 1-1e-10], on observed genotypes
+ *   side 0: Xtz (N x K)  = sum_j g/H p_j + (2-g)/(1-H) (1-p_j)
+ *           XtX (N x K x K) = sum_j g/H^2 p_j p_j^T + (2-g)/(1-H)^2 (1-p_j)(1-p_j)^T
+ *   side 1: Xtz (M x K)  = sum_i (g/H - (2-g)/(1-H)) q_i
+ *           XtX (M x K x K) = sum_i (g/H^2 + (2-g)/(1-H)^2) q_i q_i^T
+ * (positive gradient convention; the reference negates before the SQP
+ * solve).  Outputs are overwritten. */
+int adx_grad_hess(int side, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, const double *Q,
+                  const double *P, double *XtX, double *Xtz, void *stream);
+
 /* --- batched K x K solvers (FP64, 1 <= K <= 64) ---------------------------- */
 
 /* Replaces torch.ops.sqp_kernel.sqp_solve_q_cuda (simplex = 1) and

@@ -588,8 +588,12 @@ def zal_extrapolate(U: np.ndarray, V: np.ndarray, x: np.ndarray, x_next: np.ndar
     return x_next - V @ alpha
 
 
-def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None, history=None):
-    """br_qn.py:73-205 optimize_original (CPU semantics, packed G)."""
+def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None, history=None,
+         gpu_semantics=False):
+    """br_qn.py:73-205 optimize_original (CPU semantics, packed G), or with
+    ``gpu_semantics`` br_qn_gpu.py:217-382, which maps the quasi-Newton point
+    by clamp + row renormalisation (_mapPQ_gpu, :47-61) instead of the box
+    clip + simplex projection of br_qn.py:159-160."""
     P = _f64(P).copy()
     Q = _f64(Q).copy()
     N, K = Q.shape
@@ -620,8 +624,12 @@ def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None
         xq = zal_extrapolate(U[:, :nc], V[:, :nc], x, x1)
         Pq = np.clip(xq[:M * K].reshape(M, K), 1e-5, 1 - 1e-5)
         Qq = xq[M * K:].reshape(N, K).copy()
-        for i in range(N):
-            Qq[i] = project_simplex(Qq[i])
+        if gpu_semantics:
+            Qq = np.clip(Qq, 1e-5, 1 - 1e-5)
+            Qq /= Qq.sum(axis=1, keepdims=True)
+        else:
+            for i in range(N):
+                Qq[i] = project_simplex(Qq[i])
         llq = loglik_gpu(Gp, Pq, Qq, M)
         if llq > ll_prev:
             P, Q, ll_new = Pq, Qq, llq

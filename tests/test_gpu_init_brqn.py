@@ -1,0 +1,113 @@
+"""Randomized SVD, ALS, gradient/Hessian and BR-QN on the GPU against the
+oracle and the reference's demo fixtures (adamixture/demo/outputs)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from adamixture_amd import reader
+from adamixture_amd.als_gpu import ALS_gpu
+from adamixture_amd.br_qn_gpu import grad_hess, optimize_original_gpu
+from adamixture_amd.genotypes import as_packed
+from adamixture_amd.svd_gpu import SVD_gpu, xty, xw
+from adamixture_amd.utils import calculate_frequencies_gpu
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _canon(U, V):
+    U, V = U.copy(), V.copy()
+    for c in range(U.shape[1]):
+        p = np.argmax(np.abs(U[:, c]))
+        if U[p, c] < 0:
+            U[:, c] *= -1
+            V[:, c] *= -1
+    return U, V
+
+
+@pytest.fixture(scope="module")
+def demo(cuda):
+    Gt, N, M = reader.read_bed_packed(GOLDEN / "demo_data.bed")
+    return as_packed(Gt, M, cuda), N, M
+
+
+def test_svd_gemms_match_dense(cuda):
+    rng = np.random.default_rng(0)
+    M, N, L = 1037, 411, 20
+    P = rng.uniform(0.05, 0.95, size=(M, 3))
+    Q = rng.dirichlet(np.ones(3), size=N)
+    Gp = O.synth_packed(P, Q, 1, 0.03)
+    G = as_packed(torch.from_numpy(Gp), M, cuda)
+    f = O.freq_gpu(Gp, M)
+    X = O.unpack_center(Gp, f, M, 0, M).astype(np.float64)
+    W = rng.normal(size=(N, L)).astype(np.float32)
+    Y = rng.normal(size=(M, L)).astype(np.float32)
+    fd = torch.from_numpy(f).to(cuda)
+    got_y = xw(G, fd, torch.from_numpy(W).to(cuda)).cpu().numpy()
+    got_a = xty(G, fd, torch.from_numpy(Y).to(cuda)).cpu().numpy()
+    np.testing.assert_allclose(got_y, X @ W, rtol=1e-4, atol=1e-3)
+    np.testing.assert_allclose(got_a, X.T @ Y, rtol=1e-4, atol=1e-3)
+
+
+def test_svd_matches_cpu_reference_fixture(demo, fixtures):
+    G, N, M = demo
+    f = calculate_frequencies_gpu(G, M, 8192, G.device)
+    np.testing.assert_array_equal(f.cpu().numpy(), O.freq_gpu(fixtures["reader_G_packed"], M))
+    U, S, V = SVD_gpu(G, N, M, f, 7, 42, 5, 0.1, 8192, G.device)
+    U, V = _canon(U.cpu().numpy(), V.cpu().numpy())
+    np.testing.assert_allclose(S.cpu().numpy(), fixtures["cpu_brqn_svd_S"], rtol=1e-4)
+    np.testing.assert_allclose(U, fixtures["cpu_brqn_svd_U"], atol=1e-4)
+    np.testing.assert_allclose(V, fixtures["cpu_brqn_svd_V"], atol=1e-4)
+
+
+@pytest.mark.parametrize("start", ["solve", "inverse"])
+def test_als_matches_cpu_reference_fixture(demo, fixtures, start):
+    G, N, M = demo
+    f = calculate_frequencies_gpu(G, M, 8192, G.device)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(G.device)  # noqa: E731
+    P, Q = ALS_gpu(t(fixtures["cpu_brqn_svd_U"]), t(fixtures["cpu_brqn_svd_S"]), t(fixtures["cpu_brqn_svd_V"]),
+                   f, 42, M, 7, 1000, 1e-4, G.device, bvls_start=start)
+    np.testing.assert_allclose(P.cpu().numpy(), fixtures["cpu_brqn_als_P"], atol=1e-6)
+    np.testing.assert_allclose(Q.cpu().numpy(), fixtures["cpu_brqn_als_Q"], atol=1e-6)
+
+
+@pytest.mark.parametrize("K", [3, 7, 10, 16])
+def test_grad_hess_matches_oracle(cuda, K):
+    rng = np.random.default_rng(K)
+    M, N = 523, 300
+    P = rng.uniform(0.02, 0.98, size=(M, K))
+    Q = rng.dirichlet(np.full(K, 0.5), size=N)
+    Gp = O.synth_packed(P, Q, 5, 0.05)
+    G = as_packed(torch.from_numpy(Gp), M, cuda)
+    P0 = np.clip(rng.random((M, K)), 1e-5, 1 - 1e-5)
+    Q0 = rng.dirichlet(np.ones(K), size=N)
+    Pd, Qd = torch.from_numpy(P0).to(cuda), torch.from_numpy(Q0).to(cuda)
+    for side, fn in (("q", O.grad_hess_q), ("p", O.grad_hess_p)):
+        H, z = grad_hess(G, Qd, Pd, side)
+        Ho, zo = fn(Gp, Q0, P0, M)
+        np.testing.assert_allclose(H.cpu().numpy(), Ho, rtol=1e-11, atol=1e-11 * np.abs(Ho).max())
+        np.testing.assert_allclose(z.cpu().numpy(), zo, rtol=1e-11, atol=1e-11 * np.abs(zo).max())
+
+
+def test_brqn_demo_matches_oracle_and_reference_fixture(demo, fixtures):
+    G, N, M = demo
+    K = 7
+    P0, Q0 = fixtures["gpu_brqn_als_P"], fixtures["gpu_brqn_als_Q"]
+    hist = []
+    P, Q = optimize_original_gpu(G, torch.from_numpy(P0).to(G.device), torch.from_numpy(Q0).to(G.device), 10000,
+                                 K, M, N, 0.1, 3, 3, G.device, 8192, 256, on_iter=lambda it, L: hist.append(L))
+    P, Q = P.cpu().numpy(), Q.cpu().numpy()
+    ohist = []
+    Po, Qo = O.brqn(fixtures["reader_G_packed"], P0, Q0, M, history=ohist, gpu_semantics=True)
+    print(f"brqn iterations gpu {len(hist)} oracle {len(ohist)}; max|dP| {np.abs(P - Po).max():.2e} "
+          f"max|dQ| {np.abs(Q - Qo).max():.2e}; logL {hist[-1]:.6f} vs {ohist[-1][1]:.6f}")
+    assert len(hist) == len(ohist)
+    np.testing.assert_allclose(hist, [L for _, L in ohist], rtol=1e-10)
+    np.testing.assert_allclose(P, Po, atol=1e-6)
+    np.testing.assert_allclose(Q, Qo, atol=1e-6)
+    Pe, Qe = fixtures["gpu_brqn_brqn_P"], fixtures["gpu_brqn_brqn_Q"]
+    np.testing.assert_allclose(P, Pe, rtol=2e-2, atol=2e-2)
+    np.testing.assert_allclose(Q, Qe, rtol=2e-2, atol=2e-2)
+    Gu = fixtures["reader_G"]
+    assert abs(O.loglik_cpu(Gu, P, Q) - O.loglik_cpu(Gu, Pe, Qe)) <= 50.0
