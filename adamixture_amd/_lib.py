@@ -24,6 +24,7 @@ ADX_EM_LOGLIK = 2
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
 _dbl = ctypes.c_double
+_flt = ctypes.c_float
 _u64 = ctypes.c_uint64
 _p = ctypes.c_void_p
 
@@ -51,6 +52,14 @@ SIGNATURES = {
     "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
     "adx_bvls_batch": (_int, [_p, _p, _p, _p, _i64, _int, _dbl, _dbl, _p, _i64, _p]),
     "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),
+    "adx_mlp_gemm_f32": (_int, [_int, _int, _i64, _i64, _i64, _flt, _p, _i64, _p, _i64, _flt, _p, _i64, _int, _p]),
+    "adx_mlp_bias_act": (_int, [_p, _p, _i64, _i64, _i64, _int, _p]),
+    "adx_mlp_relu_bwd": (_int, [_p, _p, _i64, _p]),
+    "adx_mlp_colsum": (_int, [_p, _i64, _i64, _p, _p]),
+    "adx_mlp_softmax_xent": (_int, [_p, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p]),
+    "adx_mlp_adam": (_int, [_p, _p, _p, _p, _i64, _flt, _flt, _flt, _flt, _int, _p]),
+    "adx_mlp_pack_bf16": (_int, [_p, _p, _p]),
+    "adx_mlp_infer_bf16": (_int, [_p, _i64, _i64, _int, _p, _p, _i64, _p]),
 }
 
 _lock = threading.Lock()

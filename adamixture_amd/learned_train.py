@@ -1,0 +1,215 @@
+"""Training of the learned Q warm start on synthetic runs that the plain
+method converges (project extension, no counterpart in the reference).
+
+Each training set is drawn from the Balding-Nichols admixture model
+(synth.draw_params) with N samples, M SNPs, K in [k_lo, k_hi], F_st in
+[fst_lo, fst_hi] and Dirichlet(alpha) ancestry, alpha log-uniform in
+[alpha_lo, alpha_hi]; genotypes are generated on the GPU.  The plain method
+(randomized SVD + ALS, then Adam-EM with the reference defaults,
+adamixture/entry.py:35-47,63-72, until its own stopping rule) gives the
+target Q*; the ALS start keeps the label order of Q*, so (features, Q*) are
+aligned row by row.  Components are permuted at random per row during
+training (the network must not depend on the arbitrary cluster order).
+
+    python -m adamixture_amd.learned_train --datasets 200 --out weights.npz
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import logging
+import math
+import time
+
+import numpy as np
+import torch
+
+from .adamixture import _require_gpu
+from .als_gpu import ALS_gpu
+from .em_adam_gpu import em_pass, optimize_parameters_gpu
+from .genotypes import allele_frequencies
+from .learned_init import D_OUT, WarmStartMLP, features
+from .svd_gpu import SVD_gpu
+from .synth import Workload, make_genotypes
+
+log = logging.getLogger(__name__)
+
+# Reference defaults of the Adam-EM optimizer (adamixture/entry.py:35-47,63-72).
+ADAM_DEFAULTS = dict(lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, lr_decay=0.5, min_lr=1e-4, patience=3,
+                     tol_adam=0.1, max_iter=10000, check=5)
+SVD_DEFAULTS = dict(power=5, tol_svd=0.1, max_als=1000, tol_als=1e-4)
+
+
+def draw_config(rng: np.random.Generator, k_lo=4, k_hi=12, fst_lo=0.02, fst_hi=0.2, alpha_lo=0.1, alpha_hi=1.0):
+    K = int(rng.integers(k_lo, k_hi + 1))
+    fst = float(rng.uniform(fst_lo, fst_hi))
+    alpha = float(math.exp(rng.uniform(math.log(alpha_lo), math.log(alpha_hi))))
+    return K, fst, alpha
+
+
+def plain_start(G, f, K: int, seed: int, device, *, k_svd: int | None = None):
+    """Randomized SVD at rank k_svd (default 2K) and ALS on its first K
+    components: returns (U, S, V, P_als, Q_als)."""
+    k_svd = 2 * K if k_svd is None else k_svd
+    U, S, V = SVD_gpu(G, G.N, G.M, f, k_svd, seed, SVD_DEFAULTS["power"], SVD_DEFAULTS["tol_svd"], 0, device)
+    P_als, Q_als = ALS_gpu(U[:, :K].contiguous(), S[:K].contiguous(), V[:, :K].contiguous(), f, seed, G.M, K,
+                           SVD_DEFAULTS["max_als"], SVD_DEFAULTS["tol_als"], device)
+    return U, S, V, P_als, Q_als
+
+
+def converge_plain(G, P, Q, K: int, device, precision: str = "f64", on_check=None):
+    """Adam-EM from (P, Q) with the reference defaults until it stops."""
+    a = ADAM_DEFAULTS
+    return optimize_parameters_gpu(G, P.clone(), Q.clone(), a["lr"], a["beta1"], a["beta2"], a["reg_adam"],
+                                   a["max_iter"], a["check"], K, G.M, G.N, a["lr_decay"], a["min_lr"], a["patience"],
+                                   a["tol_adam"], device, 0, 0, precision, on_check)
+
+
+def make_training_set(seed: int, N: int, M: int, K: int, fst: float, alpha: float, miss: float, device,
+                      precision: str = "f64"):
+    """(X, T, info) of one synthetic data set: features N x 64, targets
+    N x 16 (converged Q, zero-padded)."""
+    dev = torch.device(device)
+    w = Workload(f"train{seed}", N, M, K, "bn", alpha, miss, fst=fst, seed=seed)
+    t0 = time.time()
+    G, _, _ = make_genotypes(w, dev)
+    f = allele_frequencies(G)
+    U, S, V, P_als, Q_als = plain_start(G, f, K, seed, dev)
+    X = features(U, S, V, f, P_als, Q_als, K)
+    n_checks = []
+    P_star, Q_star = converge_plain(G, P_als, Q_als, K, dev, precision, on_check=lambda it, ll: n_checks.append(it))
+    T = torch.zeros((N, D_OUT), dtype=torch.float32, device=dev)
+    T[:, :K] = Q_star.to(torch.float32)
+    ll_als = em_pass(G, P_als, Q_als, loglik=True, stats=False).item()
+    ll_star = em_pass(G, P_star, Q_star, loglik=True, stats=False).item()
+    info = dict(seed=seed, K=K, fst=fst, alpha=alpha, ll_als=ll_als, ll_star=ll_star,
+                iters=n_checks[-1] if n_checks else 0, seconds=time.time() - t0)
+    del G
+    return X, T, info
+
+
+def _permute_components(X: torch.Tensor, T: torch.Tensor, kv: torch.Tensor, gen: torch.Generator):
+    """Random permutation of the first K components of each row, applied to
+    the ALS Q block, the least-squares Q block and the target together."""
+    B = X.shape[0]
+    cols = torch.arange(D_OUT, device=X.device).expand(B, D_OUT)
+    key = torch.rand((B, D_OUT), device=X.device, generator=gen) + (cols >= kv[:, None]).to(torch.float32) * 2.0
+    perm = torch.argsort(key, dim=1)
+    X = X.clone()
+    X[:, :D_OUT] = torch.gather(X[:, :D_OUT], 1, perm)
+    X[:, D_OUT:2 * D_OUT] = torch.gather(X[:, D_OUT:2 * D_OUT], 1, perm)
+    return X, torch.gather(T, 1, perm)
+
+
+def soft_xent(Qp: torch.Tensor, T: torch.Tensor) -> float:
+    """Mean over rows of -sum t log q (zero entries of q skipped where t = 0)."""
+    lq = torch.log(Qp.clamp_min(1e-30))
+    return float(-(T * lq).sum(dim=1).mean())
+
+
+def evaluate(model: WarmStartMLP, X: torch.Tensor, T: torch.Tensor, kv: torch.Tensor) -> dict:
+    """Cross-entropy and mean |Q - Q*| of the ALS prior, the FP32 network and
+    the BF16 inference kernel on (X, T)."""
+    prior = X[:, :D_OUT].clamp_min(1e-5) * (torch.arange(D_OUT, device=X.device)[None, :] < kv[:, None])
+    prior = prior / prior.sum(dim=1, keepdim=True)
+    out = {"xent_prior": soft_xent(prior, T), "mae_prior": float((prior - T).abs().sum(1).mean())}
+    P32 = torch.cat([model.probs(X[i:i + 65536], kv[i:i + 65536]) for i in range(0, X.shape[0], 65536)])
+    out["xent_fp32"] = soft_xent(P32, T)
+    out["mae_fp32"] = float((P32 - T).abs().sum(1).mean())
+    mae_bf16, xent_bf16 = [], []
+    for K in torch.unique(kv).tolist():
+        sel = kv == K
+        Qb = model.infer(X[sel], int(K)).to(torch.float32)
+        Tk = T[sel][:, :K]
+        mae_bf16.append(float((Qb - Tk).abs().sum(1).sum()))
+        xent_bf16.append(float(-(Tk * torch.log(Qb.clamp_min(1e-30))).sum()))
+    out["mae_bf16"] = sum(mae_bf16) / X.shape[0]
+    out["xent_bf16"] = sum(xent_bf16) / X.shape[0]
+    return out
+
+
+def train(model: WarmStartMLP, X: torch.Tensor, T: torch.Tensor, kv: torch.Tensor, *, epochs: int = 40,
+          batch: int = 2048, lr: float = 1e-3, seed: int = 0, log_every: int = 5) -> list[dict]:
+    """Adam on the mean soft cross-entropy, cosine learning-rate decay,
+    random component permutations per row."""
+    gen = torch.Generator(device=X.device)
+    gen.manual_seed(seed)
+    n = X.shape[0]
+    steps_per_epoch = max(1, n // batch)
+    total = epochs * steps_per_epoch
+    hist = []
+    step = 0
+    for ep in range(epochs):
+        order = torch.randperm(n, device=X.device, generator=gen)
+        acc = torch.zeros((), dtype=torch.float32, device=X.device)
+        for s in range(steps_per_epoch):
+            idx = order[s * batch:(s + 1) * batch]
+            xb, tb = _permute_components(X[idx], T[idx], kv[idx], gen)
+            acc += model.loss_and_grad(xb, tb, kv[idx].contiguous())
+            lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
+            model.adam_step(lr_t)
+            step += 1
+        rec = {"epoch": ep + 1, "loss": float(acc) / steps_per_epoch}
+        hist.append(rec)
+        if (ep + 1) % log_every == 0 or ep == 0:
+            print(f"[learned] epoch {ep + 1}/{epochs} loss {rec['loss']:.5f}", flush=True)
+    return hist
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--datasets", type=int, default=200)
+    ap.add_argument("--val", type=int, default=20, help="held-out data sets (from the total)")
+    ap.add_argument("--N", type=int, default=5000)
+    ap.add_argument("--M", type=int, default=50000)
+    ap.add_argument("--miss", type=float, default=0.01)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
+    ap.add_argument("--budget-s", type=float, default=0.0, help="stop generating data sets after this many seconds")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    dev = _require_gpu("cuda:0")
+    rng = np.random.default_rng(args.seed)
+    Xs, Ts, Ks, infos = [], [], [], []
+    t_start = time.time()
+    for d in range(args.datasets):
+        K, fst, alpha = draw_config(rng)
+        X, T, info = make_training_set(args.seed + d, args.N, args.M, K, fst, alpha, args.miss, dev, args.precision)
+        Xs.append(X)
+        Ts.append(T)
+        Ks.append(torch.full((args.N,), K, dtype=torch.int32, device=dev))
+        infos.append(info)
+        print(f"[learned] set {d + 1}/{args.datasets} K={K} fst={fst:.3f} alpha={alpha:.2f} "
+              f"ll_als={info['ll_als']:.1f} ll*={info['ll_star']:.1f} iters={info['iters']} "
+              f"{info['seconds']:.1f}s", flush=True)
+        if args.budget_s and time.time() - t_start > args.budget_s:
+            print(f"[learned] data budget reached after {d + 1} sets", flush=True)
+            break
+    n_sets = len(Xs)
+    n_val = min(args.val, max(1, n_sets // 10))
+    tr = slice(0, n_sets - n_val)
+    va = slice(n_sets - n_val, n_sets)
+    Xtr, Ttr, Ktr = torch.cat(Xs[tr]), torch.cat(Ts[tr]), torch.cat(Ks[tr])
+    Xva, Tva, Kva = torch.cat(Xs[va]), torch.cat(Ts[va]), torch.cat(Ks[va])
+    model = WarmStartMLP(dev, seed=args.seed)
+    before = evaluate(model, Xva, Tva, Kva)
+    t0 = time.time()
+    hist = train(model, Xtr, Ttr, Ktr, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    after = evaluate(model, Xva, Tva, Kva)
+    print(f"[learned] validation before {json.dumps(before)}", flush=True)
+    print(f"[learned] validation after  {json.dumps(after)}", flush=True)
+    meta = dict(n_sets=n_sets, n_val=n_val, N=args.N, M=args.M, miss=args.miss, epochs=args.epochs,
+                batch=args.batch, lr=args.lr, seed=args.seed, precision=args.precision, train_seconds=t_train,
+                val_before=before, val_after=after, history=hist[-5:], sets=infos)
+    model.save(args.out, meta)
+    print(f"[learned] wrote {args.out} ({t_train:.1f}s training, {n_sets} sets)", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -179,6 +179,59 @@ int adx_bvls_batch(const double *A, const double *Ainv, const double *B, double 
  * accumulators.  Used to pin the lane mappings. */
 int adx_probe_mfma(int kind, const void *A, const void *B, void *D, void *stream);
 
+/* --- learned Q warm start (project extension) ----------------------------
+ * No counterpart in the reference: it initialises Q by randomized SVD + ALS
+ * only (adamixture/src/adamixture.py:185-250, model/als_gpu.py:45-127), and
+ * that path stays the fallback.  A 3-layer MLP, ADX_MLP_IN -> ADX_MLP_WIDTH
+ * -> ADX_MLP_WIDTH -> ADX_MLP_OUT with ReLU, maps per-sample features (first
+ * ADX_MLP_OUT columns: the ALS Q row, zero-padded) to logits z; the warm
+ * start is softmax(z + log max(x[:16], 1e-5)) over the first K columns.
+ * Flat FP32 parameters: W1 [IN][W] | b1 [W] | W2 [W][W] | b2 [W] |
+ * W3 [W][OUT] | b3 [OUT]  (ADX_MLP_NPARAM floats). */
+enum {
+    ADX_MLP_IN = 64,
+    ADX_MLP_WIDTH = 256,
+    ADX_MLP_OUT = 16,
+    ADX_MLP_NPARAM = 86544,
+    ADX_MLP_PACKED_BYTES = 174144
+};
+
+/* Training GEMM, FP32: C = alpha op(A) op(B) + beta C, op(X) = X or X^T
+ * (ta / tb), op(A) m x k, op(B) k x n, row-major strides.  splits > 1 splits
+ * k over workgroups with atomic accumulation (needs beta = 1). */
+int adx_mlp_gemm_f32(int ta, int tb, int64_t m, int64_t n, int64_t k, float alpha, const float *A, int64_t lda,
+                     const float *B, int64_t ldb, float beta, float *C, int64_t ldc, int splits, void *stream);
+
+/* Z[r, c] += b[c] (c < cols), then ReLU if relu != 0. */
+int adx_mlp_bias_act(float *Z, const float *b, int64_t rows, int64_t cols, int64_t ld, int relu, void *stream);
+
+/* dY[e] = 0 where Z[e] <= 0 (backward of ReLU, Z = its output). */
+int adx_mlp_relu_bwd(float *dY, const float *Z, int64_t n, void *stream);
+
+/* db[c] += sum_r dY[r, c] (atomic; db zeroed by the caller). */
+int adx_mlp_colsum(const float *dY, int64_t rows, int64_t cols, float *db, void *stream);
+
+/* Masked softmax cross-entropy over the first kvec[r] of ADX_MLP_OUT logits:
+ * z = logits + log(max(prior, 1e-5)), p = softmax(z), loss[r] = -sum t log p,
+ * dlogits = (p - t) / rows.  target, probs, dlogits and loss may be NULL. */
+int adx_mlp_softmax_xent(const float *logits, int64_t ldz, const float *prior, int64_t ldp, const float *target,
+                         int64_t ldt, const int *kvec, int64_t rows, float *probs, float *dlogits, float *loss,
+                         void *stream);
+
+/* Adam on n FP32 parameters, step t >= 1 (bias-corrected). */
+int adx_mlp_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr, float beta1, float beta2,
+                 float eps, int t, void *stream);
+
+/* Round the FP32 parameters to the BF16 inference layout
+ * (ADX_MLP_PACKED_BYTES, 16-byte aligned). */
+int adx_mlp_pack_bf16(const float *params, void *packed, void *stream);
+
+/* Fused BF16 inference (MFMA, FP32 accumulation): Q[i, :K] (FP64) = the warm
+ * start of sample i, clamped to [1e-5, 1-1e-5] and renormalised as the
+ * reference maps Q (adamixture/model/als_gpu.py:23-36).  2 <= K <= 16. */
+int adx_mlp_infer_bf16(const float *X, int64_t ldx, int64_t N, int K, const void *packed, double *Q, int64_t ldq,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -272,16 +272,28 @@ class AdamEMGPU:
 
 
 def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_iter=10000,
-               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None):
+               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None, noise=None):
     """em_adam_gpu.py:281-356 _optimize_parameters_gpu_once.
 
     Returns (P_best, Q_best).  If ``history`` is a list, (iteration, logL)
     pairs of every check are appended to it.  The third priming call of the
     reference (:322) only refills the EM target buffers, so it is omitted.
+    ``noise = (seed, amp)`` multiplies every EM target by (1 + amp z),
+    z ~ N(0, 1): with amp ~ 1e-16 it models the rounding differences of
+    another FP64 summation order, to measure how far the trajectory moves
+    under them (the chaos floor a GPU run is judged against).
     """
     P = _f64(P).copy()
     Q = _f64(Q).copy()
     N, K = Q.shape
+    rng = None if noise is None else np.random.default_rng(noise[0])
+
+    def perturb(PE, QE):
+        if rng is None:
+            return PE, QE
+        return (PE * (1.0 + noise[1] * rng.standard_normal(PE.shape)),
+                QE * (1.0 + noise[1] * rng.standard_normal(QE.shape)))
+
     opt = AdamEMGPU(M, N, K, lr, beta1, beta2, reg_adam)
     PE, QE = opt.run_em_step(Gp, P, Q, M)
     opt.step(P, Q, PE, QE)
@@ -291,7 +303,7 @@ def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, 
     P_best, Q_best = P.copy(), Q.copy()
     wait = 0
     for it in range(max_iter):
-        PE, QE = opt.run_em_step(Gp, P, Q, M)
+        PE, QE = perturb(*opt.run_em_step(Gp, P, Q, M))
         opt.step(P, Q, PE, QE)
         if (it + 1) % check == 0:
             L = loglik_gpu(Gp, P, Q, M)

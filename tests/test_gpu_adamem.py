@@ -1,12 +1,15 @@
 """Adam-EM trajectories on the GPU against the oracle and the reference's own
 fixtures (adamixture/demo/outputs/gpu/adamem, CUDA FP64 run).
 
-Over hundreds of Adam-EM iterations the trajectory amplifies rounding: the
-oracle run twice, from inits that differ by 1e-15 relative, ends ~4e-6 apart
-in P on the 1,000 x 20,000 workload.  The FP64 GPU run is therefore required
-to stay within 10x of that measured sensitivity (and to take the same
-lr-decay/stop decisions), while single steps are pinned to 1e-11 in
-test_gpu_kernels.py."""
+Over hundreds of Adam-EM iterations the trajectory amplifies rounding.  The
+chaos floor is measured on the oracle itself: it is run again with every EM
+target multiplied by (1 + 1e-16 z), z ~ N(0, 1) -- the size of the rounding
+differences another FP64 summation order makes -- for two noise seeds.  On
+the reference demo data this moves P by ~2e-3 and logL by ~4e-8 relative at
+the end (perturbing only the initial P by 1e-15 moves it ~500x less and
+underestimates the floor).  The FP64 GPU run must stay within 3x of the
+larger of the two noisy runs and take the same check iterations, while
+single steps are pinned to 1e-11 in test_gpu_kernels.py."""
 import numpy as np
 import pytest
 import torch
@@ -24,13 +27,21 @@ HP = dict(lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, check=5, lr_decay=0.5
 
 
 def _sensitivity(Gp, P0, Q0, M, max_iter, Po, Qo, ohist):
-    """How far the oracle moves when its init is perturbed by 1e-15 relative."""
-    P1 = P0 * (1.0 + 1e-15 * np.random.default_rng(1).standard_normal(P0.shape))
-    h = []
+    """How far the oracle moves under FP64-rounding-sized noise on its EM
+    targets (max over two noise seeds): P, Q at the end, and the largest
+    relative logL difference over the checks both runs share."""
     kw = {} if max_iter is None else {"max_iter": max_iter}
-    Pp, Qp = O.adamem_gpu(Gp, P1, Q0, M, history=h, **kw)
-    return {"P": np.abs(Pp - Po).max(), "Q": np.abs(Qp - Qo).max(),
-            "ll": abs(h[-1][1] - ohist[-1][1]) / abs(ohist[-1][1])}
+    out = {"P": 0.0, "Q": 0.0, "ll": 0.0}
+    L0 = np.array([L for _, L in ohist])
+    for seed in (0, 1):
+        h = []
+        Pp, Qp = O.adamem_gpu(Gp, P0, Q0, M, history=h, noise=(seed, 1e-16), **kw)
+        L1 = np.array([L for _, L in h])
+        n = min(len(L0), len(L1))
+        out["P"] = max(out["P"], np.abs(Pp - Po).max())
+        out["Q"] = max(out["Q"], np.abs(Qp - Qo).max())
+        out["ll"] = max(out["ll"], float(np.max(np.abs(L1[:n] - L0[:n]) / np.abs(L0[:n]))))
+    return out
 
 
 def _run(G, P0, Q0, M, N, K, dev, precision, max_iter=10000):
@@ -53,9 +64,10 @@ def test_demo_adamem_f64_matches_oracle_and_reference_fixture(cuda, fixtures):
           np.abs(Q - Qo).max(), "last logL", hist[-1], ohist[-1])
     assert [it for it, _ in hist] == [it for it, _ in ohist]      # same lr schedule and stop
     floor = _sensitivity(fixtures["reader_G_packed"], P0, Q0, M, None, Po, Qo, ohist)
-    np.testing.assert_allclose([L for _, L in hist], [L for _, L in ohist], rtol=max(1e-11, 10 * floor["ll"]))
-    assert np.abs(P - Po).max() <= 10 * floor["P"] + 1e-12
-    assert np.abs(Q - Qo).max() <= 10 * floor["Q"] + 1e-12
+    print(f"oracle chaos floor: dP {floor['P']:.3e} dQ {floor['Q']:.3e} logL rel {floor['ll']:.3e}")
+    np.testing.assert_allclose([L for _, L in hist], [L for _, L in ohist], rtol=max(1e-11, 3 * floor["ll"]))
+    assert np.abs(P - Po).max() <= 3 * floor["P"] + 1e-12
+    assert np.abs(Q - Qo).max() <= 3 * floor["Q"] + 1e-12
     # and the reference's own CUDA fixture at its tolerance (tests/config.py:24-26)
     Pe, Qe = fixtures["gpu_adamem_adamem_P"], fixtures["gpu_adamem_adamem_Q"]
     np.testing.assert_allclose(P, Pe, rtol=2e-2, atol=2e-2)
@@ -84,9 +96,9 @@ def test_cpu_ref_workload_fixed_iterations(cuda, precision):
     if precision == "f64":
         assert [it for it, _ in hist] == [it for it, _ in ohist]
         floor = _sensitivity(G.to_host_packed(), P0, Q0, w.M, 100, Po, Qo, ohist)
-        print(f"oracle self-sensitivity: dP {floor['P']:.3e} dQ {floor['Q']:.3e} logL rel {floor['ll']:.3e}")
-        assert dP <= 10 * floor["P"] + 1e-12 and dQ <= 10 * floor["Q"] + 1e-12
-        assert rel_ll <= max(1e-11, 10 * floor["ll"])
+        print(f"oracle chaos floor: dP {floor['P']:.3e} dQ {floor['Q']:.3e} logL rel {floor['ll']:.3e}")
+        assert dP <= 3 * floor["P"] + 1e-12 and dQ <= 3 * floor["Q"] + 1e-12
+        assert rel_ll <= max(1e-11, 3 * floor["ll"])
     else:
         assert dP < 5e-3 and dQ < 5e-3 and rel_ll < 1e-6
     Gl = G

@@ -65,9 +65,12 @@ def test_mfma_bf16_lane_layout(cuda):
     """16x16x32 BF16 MFMA: operands A[l%16][8(l/16)+i], B[8(l/16)+i][l%16],
     accumulator element j of lane l = D[4(l/16)+j][l%16]."""
     rng = np.random.default_rng(2)
-    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)      # exact in bf16
+    A = rng.integers(-8, 9, size=(16, 32)).astype(np.float32)
     B = rng.integers(-8, 9, size=(32, 16)).astype(np.float32)
     A[np.arange(16), np.arange(16)] += 50.0 * (1 + np.arange(16))   # make rows of A @ B distinct
+    # the probe rounds its FP32 inputs to BF16 (the diagonal above 256 is not
+    # exact in 8 bits); integer products and sums are exact in FP32
+    A = torch.from_numpy(A).to(torch.bfloat16).to(torch.float32).numpy()
     want = A.astype(np.float64) @ B.astype(np.float64)
     raw = torch.zeros(256, dtype=torch.float32, device=cuda)
     a, b = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda)

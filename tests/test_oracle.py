@@ -360,3 +360,110 @@ def test_brqn_matches_reference_fixture(fixtures):
     P, Q = O.brqn(Gp, fixtures["cpu_brqn_als_P"], fixtures["cpu_brqn_als_Q"], M)
     np.testing.assert_allclose(P, fixtures["cpu_brqn_brqn_P"], atol=1e-8)
     np.testing.assert_allclose(Q, fixtures["cpu_brqn_brqn_Q"], atol=1e-8)
+
+
+# ------------------------------------------------------- learned warm start ----
+
+def _mlp_case(seed=3, rows=7):
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(MO.NPARAM) * 0.08
+    X = rng.random((rows, MO.D0))
+    X[:, :MO.D3] = rng.dirichlet(np.ones(MO.D3), size=rows)
+    kvec = rng.integers(2, MO.D3 + 1, size=rows)
+    T = np.zeros((rows, MO.D3))
+    for r, k in enumerate(kvec):
+        T[r, :k] = rng.dirichlet(np.ones(k))
+    return MO, w, X, kvec, T
+
+
+def test_mlp_oracle_gradient_finite_differences():
+    """Every parameter block, including ReLU units on both sides of zero:
+    central differences of the loss match backward() (kink-free points)."""
+    MO, w, X, kvec, T = _mlp_case()
+    g = MO.backward(w, X, kvec, T)
+    rng = np.random.default_rng(0)
+    off = 0
+    checked = 0
+    for name, shp in MO.SHAPES:
+        n = int(np.prod(shp))
+        for idx in rng.choice(n, size=12, replace=False):
+            j = off + idx
+            e = 1e-6
+            wp, wm = w.copy(), w.copy()
+            wp[j] += e
+            wm[j] -= e
+            fd = (MO.loss(wp, X, kvec, T) - MO.loss(wm, X, kvec, T)) / (2 * e)
+            assert abs(fd - g[j]) <= 1e-7 + 1e-5 * abs(g[j]), (name, idx, fd, g[j])
+            checked += 1
+        off += n
+    assert checked == 72
+    _, (_, h1, h2, _, _) = MO.forward(w, X, kvec)
+    assert (h1 == 0).any() and (h1 > 0).any() and (h2 == 0).any() and (h2 > 0).any()
+
+
+def test_mlp_oracle_softmax_closed_form():
+    """With zero weights the output is the renormalised prior and the
+    gradient w.r.t. b3 is mean(p - t) over the rows (masked columns zero)."""
+    MO, w, X, kvec, T = _mlp_case(seed=5)
+    w = np.zeros_like(w)
+    p, _ = MO.forward(w, X, kvec)
+    for r, k in enumerate(kvec):
+        prior = np.maximum(X[r, :k], 1e-5)
+        np.testing.assert_allclose(p[r, :k], prior / prior.sum(), rtol=1e-12)
+        assert np.all(p[r, k:] == 0)
+    g = MO.unflatten(MO.backward(w, X, kvec, T))
+    mask = np.arange(MO.D3)[None, :] < kvec[:, None]
+    np.testing.assert_allclose(g["b3"], np.where(mask, p - T, 0).mean(0), atol=1e-15)
+    assert np.all(g["W1"] == 0) and np.all(g["W2"] == 0)
+
+
+def test_mlp_oracle_adam_textbook():
+    """Two Adam steps against Kingma & Ba, Algorithm 1, written out by hand;
+    the first bias-corrected step moves each weight by lr sign(g)."""
+    from oracle import mlp_oracle as MO
+    g1, g2 = np.array([0.3, -2.0, 1e-3]), np.array([-0.1, 0.5, 0.0])
+    w0 = np.array([1.0, 2.0, 3.0])
+    w1, m, v = MO.adam(w0, g1, np.zeros(3), np.zeros(3), 1, 0.01)
+    np.testing.assert_allclose(w1, w0 - 0.01 * g1 / (np.abs(g1) + 1e-8), rtol=1e-12)
+    w2, m, v = MO.adam(w1, g2, m, v, 2, 0.01)
+    m_ref = 0.9 * 0.1 * g1 + 0.1 * g2
+    v_ref = 0.999 * 0.001 * g1 ** 2 + 0.001 * g2 ** 2
+    np.testing.assert_allclose(w2, w1 - 0.01 * (m_ref / (1 - 0.81)) / (np.sqrt(v_ref / (1 - 0.998001)) + 1e-8),
+                               rtol=1e-12)
+
+
+def test_mlp_oracle_bf16_rounding():
+    """Round-to-nearest-even on the 16-bit boundary, ties to even."""
+    from oracle import mlp_oracle as MO
+    one = np.float32(1.0)
+    ulp = np.float32(2.0 ** -7)
+    x = np.array([one, one + ulp / 2, one + ulp * 1.5, one + ulp * 0.75, -3.0], np.float32)
+    np.testing.assert_array_equal(MO._bf16(x), [1.0, 1.0, 1.0 + 2 * float(ulp), 1.0 + float(ulp), -3.0])
+
+
+def test_learned_features_host_logic():
+    """Feature layout of the warm start on a tiny exact case (torch on the
+    CPU): ALS prior block, the unconstrained least-squares block reproduces
+    Q when the low-rank model is exact, energy fraction, K encodings."""
+    import torch
+    from adamixture_amd.learned_init import D_IN, features
+    rng = np.random.default_rng(1)
+    M, N, K = 60, 40, 3
+    P = rng.uniform(0.1, 0.9, (M, K))
+    Q = rng.dirichlet(np.ones(K), size=N)
+    X = P @ Q.T                                   # expected genotype / 2
+    f = X.mean(axis=1)
+    C = 2 * X - 2 * f[:, None]                    # centred expected genotypes
+    U, S, Vt = np.linalg.svd(C, full_matrices=False)
+    r = 2 * K
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a))  # noqa: E731
+    F = features(t(U[:, :r]), t(S[:r]), t(Vt[:r].T), t(f), t(P), t(Q), K).numpy()
+    assert F.shape == (N, D_IN) and F.dtype == np.float32
+    np.testing.assert_allclose(F[:, :K], Q, rtol=1e-6)
+    assert np.all(F[:, K:16] == 0)
+    np.testing.assert_allclose(F[:, 16:16 + K], Q, atol=1e-5)   # exact model: LS returns Q
+    assert np.all(F[:, 32] < 1e-6)                # rank-(K-1) centred data: no tail energy
+    np.testing.assert_allclose(F[:, 34], K / 16)
+    np.testing.assert_allclose(F[:, 35], 1 / K)
+    np.testing.assert_allclose(F[0, 36:36 + r - 1], (S[1:r] / S[0]).astype(np.float32), rtol=1e-5)
