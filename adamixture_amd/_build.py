@@ -20,6 +20,8 @@ HIPCC_FLAGS = [
     "-munsafe-fp-atomics",
     # packed FP32 VALU ops are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs (no AGPR round trips in the elementwise epilogues)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

@@ -33,6 +33,16 @@ int fail_msg(int code, const std::string& msg);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Tile configuration of the EM passes for one dtype tag (ADX_F32, ADX_F64,
+// ADX_BF16): SNP column tiles per wave and the target number of workgroups
+// (the sample range is split until the grid reaches it).  Process-wide, set
+// by adx_set_em_tiles (the autotuner); read at every launch.
+struct EmTiles {
+    int cw;
+    int64_t target_wgs;
+};
+EmTiles em_tiles(int dtype);
+
 // One 16x16x4 MFMA step.  Operand layouts are shared by FP32 and FP64:
 //   A (16x4):  lane l supplies A[l % 16][l / 16]
 //   B (4x16):  lane l supplies B[l / 16][l % 16]

@@ -319,9 +319,18 @@ __global__ void finalize_q_kernel(const double* __restrict__ Q, TA* __restrict__
 }
 
 // em_adam_gpu.py:11-42 + :222 (P side), elementwise.
+// With sched != NULL the step's lr and bias corrections come from device
+// memory (sched = [t, lr, 1 - b1^t, 1 - b2^t], advanced by sched_tick_kernel)
+// so that the step can be replayed from a captured graph.
 __global__ void adam_p_kernel(double* __restrict__ P, const double* __restrict__ PE, double* __restrict__ m,
                               double* __restrict__ v, int64_t M, int K, double bc1, double bc2, double lr,
-                              double b1, double b2, double eps, float* __restrict__ P_lo, int64_t ld_lo) {
+                              double b1, double b2, double eps, float* __restrict__ P_lo, int64_t ld_lo,
+                              const double* __restrict__ sched) {
+    if (sched) {
+        lr = sched[1];
+        bc1 = sched[2];
+        bc2 = sched[3];
+    }
     const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= M * K) return;
     const double d = PE[x] - P[x];
@@ -338,7 +347,13 @@ __global__ void adam_p_kernel(double* __restrict__ P, const double* __restrict__
 // em_adam_gpu.py:11-42 + :223-224 (Q side), one thread per sample.
 __global__ void adam_q_kernel(double* __restrict__ Q, const double* __restrict__ QE, double* __restrict__ m,
                               double* __restrict__ v, int64_t N, int K, double bc1, double bc2, double lr,
-                              double b1, double b2, double eps, float* __restrict__ Q_lo, int64_t ld_lo) {
+                              double b1, double b2, double eps, float* __restrict__ Q_lo, int64_t ld_lo,
+                              const double* __restrict__ sched) {
+    if (sched) {
+        lr = sched[1];
+        bc1 = sched[2];
+        bc2 = sched[3];
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     double s = 0.0;
@@ -361,23 +376,35 @@ __global__ void adam_q_kernel(double* __restrict__ Q, const double* __restrict__
     }
 }
 
+// t += 1 and the bias corrections of step t (EMAdamOptimizer.step increments
+// t before use, em_adam_gpu.py:201-210).
+__global__ void sched_tick_kernel(double* __restrict__ sched, double b1, double b2) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double t = sched[0] + 1.0;
+        sched[0] = t;
+        sched[2] = 1.0 - pow(b1, t);
+        sched[3] = 1.0 - pow(b2, t);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side dispatch
 // ---------------------------------------------------------------------------
+static EmTiles g_tiles[3] = {{8, 2048}, {4, 2048}, {4, 2048}};  // ADX_F32, ADX_F64, ADX_BF16
+
+EmTiles em_tiles(int dtype) { return g_tiles[dtype >= 0 && dtype < 3 ? dtype : 0]; }
+
 struct EmGrid {
+    int cw;
     int64_t nbx, splits, n_per_split;
 };
 
-template <typename T>
-constexpr int em_cw() {
-    return sizeof(T) == 4 ? 8 : 4;
-}
-
 static EmGrid em_grid(int dtype, int64_t M, int64_t N) {
-    const int cw = dtype == ADX_F32 ? em_cw<float>() : em_cw<double>();
+    const EmTiles tl = em_tiles(dtype);
     EmGrid gd;
-    gd.nbx = ceil_div(M, 64 * cw);
-    const int64_t target = 2048;
+    gd.cw = tl.cw;
+    gd.nbx = ceil_div(M, 64 * tl.cw);
+    const int64_t target = tl.target_wgs;
     int64_t s = ceil_div(target, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
     if (s > max_s) s = max_s;
@@ -393,7 +420,14 @@ static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_
                       int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B, T* Tm,
                       double* llp) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
-    em_pass_kernel<T, KS, em_cw<T>(), STATS, LL>
+    if constexpr (sizeof(T) == 4) {  // the FP64 form has a single tile width
+        if (gd.cw == 8) {
+            em_pass_kernel<T, KS, 8, STATS, LL>
+                <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp);
+            return;
+        }
+    }
+    em_pass_kernel<T, KS, 4, STATS, LL>
         <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp);
 }
 
@@ -414,6 +448,22 @@ static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int6
 using namespace adx;
 
 extern "C" {
+
+int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs) {
+    if (dtype < 0 || dtype > 2) return fail_msg(1, "adx_set_em_tiles: unknown dtype tag");
+    const bool ok_cw = dtype == ADX_F64 ? cw == 4 : (cw == 4 || cw == 8);
+    if (!ok_cw) return fail_msg(1, "adx_set_em_tiles: cw must be 4 or 8 (4 for ADX_F64)");
+    if (target_wgs < 64 || target_wgs > 65536) return fail_msg(1, "adx_set_em_tiles: target_wgs not in [64, 65536]");
+    g_tiles[dtype] = EmTiles{cw, target_wgs};
+    return 0;
+}
+
+int adx_get_em_tiles(int dtype, int* cw, int64_t* target_wgs) {
+    if (dtype < 0 || dtype > 2) return fail_msg(1, "adx_get_em_tiles: unknown dtype tag");
+    if (cw) *cw = g_tiles[dtype].cw;
+    if (target_wgs) *target_wgs = g_tiles[dtype].target_wgs;
+    return 0;
+}
 
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
     (void)K;
@@ -499,11 +549,29 @@ int adx_adam_step(double* P, double* Q, const double* P_EM, const double* Q_EM, 
     hipStream_t st = (hipStream_t)stream;
     if (M > 0)
         adam_p_kernel<<<(unsigned)ceil_div(M * K, 256), 256, 0, st>>>(P, P_EM, mP, vP, M, K, bc1, bc2, lr, beta1,
-                                                                     beta2, eps, P_lo, ld_lo);
+                                                                     beta2, eps, P_lo, ld_lo, nullptr);
     if (N > 0)
         adam_q_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(Q, Q_EM, mQ, vQ, N, K, bc1, bc2, lr, beta1, beta2,
-                                                                  eps, Q_lo, ld_lo);
+                                                                  eps, Q_lo, ld_lo, nullptr);
     ADX_LAUNCH_CHECK("adx_adam_step");
+    return 0;
+}
+
+int adx_adam_step_sched(double* P, double* Q, const double* P_EM, const double* Q_EM, double* mP, double* vP,
+                        double* mQ, double* vQ, int64_t M, int64_t N, int K, double* sched, double beta1,
+                        double beta2, double eps, float* P_lo, float* Q_lo, int64_t ld_lo, void* stream) {
+    if (K < 1) return fail_msg(1, "adx_adam_step_sched: K < 1");
+    if (!sched) return fail_msg(1, "adx_adam_step_sched: sched is NULL");
+    if ((P_lo || Q_lo) && ld_lo < K) return fail_msg(1, "adx_adam_step_sched: ld_lo < K");
+    hipStream_t st = (hipStream_t)stream;
+    sched_tick_kernel<<<1, 64, 0, st>>>(sched, beta1, beta2);
+    if (M > 0)
+        adam_p_kernel<<<(unsigned)ceil_div(M * K, 256), 256, 0, st>>>(P, P_EM, mP, vP, M, K, 0.0, 0.0, 0.0, beta1,
+                                                                     beta2, eps, P_lo, ld_lo, sched);
+    if (N > 0)
+        adam_q_kernel<<<(unsigned)ceil_div(N, 256), 256, 0, st>>>(Q, Q_EM, mQ, vQ, N, K, 0.0, 0.0, 0.0, beta1, beta2,
+                                                                  eps, Q_lo, ld_lo, sched);
+    ADX_LAUNCH_CHECK("adx_adam_step_sched");
     return 0;
 }
 

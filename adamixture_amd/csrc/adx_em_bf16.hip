@@ -10,9 +10,12 @@
 //          K <= 10, two for K <= 16
 //   A^T += (Qt_hi + Qt_lo) . a^T,  B^T += (Qt_hi + Qt_lo) . b^T
 //          k = 32 samples (two 16-sample row blocks), a, b rounded to BF16
-//   T   += [a | b]^T . [P_hi ; 1 - P_hi]      k = 16 SNPs x {a, b}; the
-//          [a | b] tile goes through a 1 KB per-wave LDS transpose and the
-//          colsum(b) term of em_batch_math is the (1 - P) half.
+//   T   += [a b] . [P_hi ; 1 - P_hi]   k = 16 SNPs x {a, b}: a and b of one
+//          genotype are rounded into one word (bf16 pair), the 16 x 16 word
+//          tile is transposed through LDS (one 32-bit write per genotype, one
+//          128-bit read per lane), and the colsum(b) term of em_batch_math
+//          is the (1 - P) half.  The same words, permuted, are the B
+//          operands of the A and B contractions.
 //
 // a = g/H and b = (2-g)/(1-H) are formed in FP32 from the FP32 accumulator,
 // with one reciprocal of H(1-H).  The operand arrays Px, Qx, Qt are rebuilt
@@ -26,19 +29,42 @@
 
 namespace adx {
 
-__device__ __forceinline__ void em_terms_f32(float h, uint32_t code, float& a, float& b) {
-    const float hc = fminf(fmaxf(h, 1e-5f), 1.0f - 1e-5f);
+// Per-byte genotype weights of four genotypes at once: w holds one 2-bit
+// code per byte (bits above cleared); returns ga = g and gb = 2 - g per byte,
+// both 0 where the code is 3 (missing).
+__device__ __forceinline__ void code_weights(uint32_t w, uint32_t& ga, uint32_t& gb) {
+    const uint32_t m = (w >> 1) & w & 0x01010101u;   // 1 in missing bytes
+    ga = w ^ (m | (m << 1));                          // 3 -> 0
+    gb = (0x02020202u ^ (m << 1)) - ga;               // 2 - g, missing -> 0 (no borrows)
+}
+
+// a = ga / H and b = gb / (1 - H) with H clamped to [1e-5, 1 - 1e-5]
+// (em_adam_gpu.py:62-72), one reciprocal of H (1 - H).
+__device__ __forceinline__ void em_terms_f32(float h, float ga, float gb, float& a, float& b) {
+    const float hc = __builtin_amdgcn_fmed3f(h, 1e-5f, 1.0f - 1e-5f);
     const float u = 1.0f - hc;
     const float r = __builtin_amdgcn_rcpf(hc * u);
-    const float ca = code == 3u ? 0.0f : (float)code;
-    const float cb = code == 3u ? 0.0f : 2.0f - (float)code;
-    a = ca * (u * r);
-    b = cb * (hc * r);
+    a = ga * (u * r);
+    b = gb * (hc * r);
+}
+
+template <int J>
+__device__ __forceinline__ float byte_to_float(uint32_t w) {
+    return (float)((w >> (8 * J)) & 0xFFu);
 }
 
 __device__ __forceinline__ bf16x8 load_bf16x8(const __bf16* p) {
     return *reinterpret_cast<const bf16x8*>(p);
 }
+
+// Two FP32 values rounded to BF16 and packed into one word (lo = x, hi = y).
+__device__ __forceinline__ uint32_t pack_bf16x2(float x, float y) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const bf16x2 v = bf16x2{(__bf16)x, (__bf16)y};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <int KH, int CW>
 __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
@@ -47,6 +73,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc) {
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
     constexpr int LX = 32 * KH;  // row length of Px / Qx
+    constexpr int TS = 20;       // row stride (words) of the per-wave transpose tile
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
@@ -57,7 +84,8 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
 
-    __shared__ __attribute__((aligned(16))) __bf16 s_tile[4][2][16][16];   // per wave: [a|b][row][SNP col]
+    // Per wave: word [sample row r][SNP column c] = bf16 a | bf16 b << 16.
+    __shared__ __attribute__((aligned(16))) uint32_t s_tile[4][16 * TS];
     __shared__ float s_tsum[64 * 16];
 
     for (int e = threadIdx.x; e < 64 * 16; e += 256) s_tsum[e] = 0.0f;
@@ -71,15 +99,16 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         const int64_t snp = snp_w + 64 * t + 4 * cl + f;
 #pragma unroll
         for (int kh = 0; kh < KH; ++kh) pH[cb][kh] = snp < M ? load_bf16x8(Px + snp * LX + 32 * kh + 8 * g) : zero8;
-        // T B-operand: k = 8g + i -> (type g >> 1, SNP column 8 (g & 1) + i), column kp = cl.
+        // T B-operand: k = 8g + i <-> (SNP column 4g + i/2, P for even i, 1 - P
+        // for odd i), matching the (a, b) pairs of the tile words; column kp = cl.
         bf16x8 v;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int64_t s2 = snp_w + 64 * t + 4 * (8 * (g & 1) + i) + f;
+            const int64_t s2 = snp_w + 64 * t + 4 * (4 * g + (i >> 1)) + f;
             float x = 0.0f;
             if (s2 < M && cl < K) {
                 const float p = (float)Px[s2 * LX + cl];
-                x = (g >> 1) ? 1.0f - p : p;
+                x = (i & 1) ? 1.0f - p : p;
             }
             v[i] = (__bf16)x;
         }
@@ -92,6 +121,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         accA[cb] = floatx4{0, 0, 0, 0};
         accB[cb] = floatx4{0, 0, 0, 0};
     }
+    uint32_t* tile = s_tile[wave];
     __syncthreads();
 
     for (int64_t s0 = n0; s0 < n1; s0 += 64) {
@@ -138,27 +168,45 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
 #pragma unroll
             for (int cb = 0; cb < CW; ++cb) {
                 const int t = cb >> 2, f = cb & 3;
-                bf16x8 aB, bB;
+                uint32_t wab[2][4];
 #pragma unroll
                 for (int rr = 0; rr < 2; ++rr) {
                     floatx4 h = floatx4{0, 0, 0, 0};
 #pragma unroll
                     for (int kh = 0; kh < KH; ++kh) h = mma_bf16(qH[rr][kh], pH[cb][kh], h);
-                    const uint32_t w = gw[t][2 * tp + rr] >> (2 * f);
+                    uint32_t ga, gb;
+                    code_weights((gw[t][2 * tp + rr] >> (2 * f)) & 0x03030303u, ga, gb);
+                    const float fa[4] = {byte_to_float<0>(ga), byte_to_float<1>(ga), byte_to_float<2>(ga),
+                                         byte_to_float<3>(ga)};
+                    const float fb[4] = {byte_to_float<0>(gb), byte_to_float<1>(gb), byte_to_float<2>(gb),
+                                         byte_to_float<3>(gb)};
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         float a, b;
-                        em_terms_f32(h[j], (w >> (8 * j)) & 3u, a, b);
-                        const __bf16 ab = (__bf16)a, bb = (__bf16)b;
-                        aB[4 * rr + j] = ab;
-                        bB[4 * rr + j] = bb;
-                        s_tile[wave][0][4 * g + j][cl] = ab;
-                        s_tile[wave][1][4 * g + j][cl] = bb;
+                        em_terms_f32(h[j], fa[j], fb[j], a, b);
+                        wab[rr][j] = pack_bf16x2(a, b);
+                        tile[(4 * g + j) * TS + cl] = wab[rr][j];
                     }
-                    // T (16 samples x 16 kp) += [a|b] (16 x 32) . [P ; 1-P] (32 x 16)
-                    const bf16x8 tv = *reinterpret_cast<const bf16x8*>(&s_tile[wave][g >> 1][cl][8 * (g & 1)]);
+                    // The tile is written and read by lanes of this wave only;
+                    // LDS operations of a wave complete in order, and the
+                    // wave barrier keeps the compiler from moving them.
+                    __builtin_amdgcn_wave_barrier();
+                    // T (16 samples x 16 kp) += [a b] (16 x 32) . [P ; 1-P] (32 x 16)
+                    const bf16x8 tv = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(tile + cl * TS + 4 * g));
+                    __builtin_amdgcn_wave_barrier();
                     accT[rr] = mma_bf16(tv, pT[cb], accT[rr]);
                 }
+                // B operands of A^T, B^T: k = sample 8g + i (rr = i / 4, j = i % 4).
+                const u32x4 wa = u32x4{__builtin_amdgcn_perm(wab[0][1], wab[0][0], 0x05040100u),
+                                       __builtin_amdgcn_perm(wab[0][3], wab[0][2], 0x05040100u),
+                                       __builtin_amdgcn_perm(wab[1][1], wab[1][0], 0x05040100u),
+                                       __builtin_amdgcn_perm(wab[1][3], wab[1][2], 0x05040100u)};
+                const u32x4 wb = u32x4{__builtin_amdgcn_perm(wab[0][1], wab[0][0], 0x07060302u),
+                                       __builtin_amdgcn_perm(wab[0][3], wab[0][2], 0x07060302u),
+                                       __builtin_amdgcn_perm(wab[1][1], wab[1][0], 0x07060302u),
+                                       __builtin_amdgcn_perm(wab[1][3], wab[1][2], 0x07060302u)};
+                const bf16x8 aB = __builtin_bit_cast(bf16x8, wa);
+                const bf16x8 bB = __builtin_bit_cast(bf16x8, wb);
                 accA[cb] = mma_bf16(qThi, aB, accA[cb]);
                 accA[cb] = mma_bf16(qTlo, aB, accA[cb]);
                 accB[cb] = mma_bf16(qThi, bB, accB[cb]);
@@ -251,15 +299,16 @@ __global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, _
 }
 
 struct Bf16Grid {
+    int cw;
     int64_t nbx, splits, n_per_split;
 };
 
-constexpr int BF16_CW = 8;
-
 static Bf16Grid bf16_grid(int64_t M, int64_t N) {
+    const EmTiles tl = em_tiles(ADX_BF16);
     Bf16Grid gd;
-    gd.nbx = ceil_div(M, 64 * BF16_CW);
-    int64_t s = ceil_div(2048, gd.nbx);
+    gd.cw = tl.cw;
+    gd.nbx = ceil_div(M, 64 * tl.cw);
+    int64_t s = ceil_div(tl.target_wgs, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
     if (s > max_s) s = max_s;
     if (s < 1) s = 1;
@@ -267,6 +316,17 @@ static Bf16Grid bf16_grid(int64_t M, int64_t N) {
     gd.n_per_split = ceil_div(ceil_div(N, s), 64) * 64;
     gd.splits = ceil_div(N, gd.n_per_split);
     return gd;
+}
+
+template <int KH>
+static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
+                        int K, const __bf16* Px, const __bf16* Qx, const __bf16* Qt, int64_t ldqt, float* A,
+                        float* B, float* T) {
+    dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
+    if (gd.cw == 8)
+        em_pass_bf16_kernel<KH, 8><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T);
+    else
+        em_pass_bf16_kernel<KH, 4><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T);
 }
 
 }  // namespace adx
@@ -320,14 +380,11 @@ int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
     const __bf16* Qx = Px + M * 32 * kh;
     const __bf16* Qt = Qx + N * 32 * kh;
     const Bf16Grid gd = bf16_grid(M, N);
-    dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     hipStream_t st = (hipStream_t)stream;
     if (kh == 1)
-        em_pass_bf16_kernel<1, BF16_CW><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A,
-                                                              B, T);
+        launch_bf16<1>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T);
     else
-        em_pass_bf16_kernel<2, BF16_CW><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A,
-                                                              B, T);
+        launch_bf16<2>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T);
     ADX_LAUNCH_CHECK("adx_em_pass_bf16");
     return 0;
 }

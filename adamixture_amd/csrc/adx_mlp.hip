@@ -287,10 +287,11 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float* __restrict_
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o);
         p = c < K ? fminf(fmaxf(p / s, 1e-5f), 1.0f - 1e-5f) : 0.0f;
-        s = p;
+        // renormalise in FP64 so that the stored row sums to 1 in FP64
+        double pd = (double)p, sd = pd;
 #pragma unroll
-        for (int o = 1; o < 16; o <<= 1) s += __shfl_xor(s, o);
-        if (live && c < K) Q[r * ldq + c] = (double)p / (double)s;
+        for (int o = 1; o < 16; o <<= 1) sd += __shfl_xor(sd, o);
+        if (live && c < K) Q[r * ldq + c] = pd / sd;
     }
 }
 

@@ -17,15 +17,18 @@ Features of sample i (row of X, FP32, 64 columns):
   [0, 16)   ALS Q row (K entries, zero-padded): the prior of the softmax
   [16, 32)  unconstrained least-squares Q row on the rank-2K SVD with the
             ALS P (the ALS Q-update before the bounds, als_gpu.py:108-113)
-  32        fraction of the sample's rank-2K score energy in components K..2K-1
-  33        log(|s_i| / median_i |s_i|), s_i = S * V[i] (rank 2K)
-  34, 35    K / 16, 1 / K
-  [36, 52)  singular-value profile S[k] / S[0], k = 1..16 (0 beyond 2K-1)
-  [52, 64)  zero
+  [32, 48)  log(max(ALS Q, 1e-5)) / log(1e-5)  (in [0, 1]; 0 beyond K)
+  48        fraction of the sample's rank-2K score energy in components K..2K-1
+  49        log(|s_i| / median_i |s_i|), s_i = S * V[i] (rank 2K)
+  50, 51    K / 16, 1 / K
+  [52, 64)  singular-value profile S[k] / S[0], k = 1..12 (0 beyond 2K-1)
+The three K-indexed blocks follow the ALS cluster order; training permutes
+them together.
 """
 from __future__ import annotations
 
 import logging
+import math
 import time
 from pathlib import Path
 
@@ -89,20 +92,22 @@ def features(U2: torch.Tensor, S2: torch.Tensor, V2: torch.Tensor, f: torch.Tens
     f = f.to(dev, dt)
     P = P_als.to(dev, dt)
     X = torch.zeros((N, D_IN), dtype=torch.float32, device=dev)
-    X[:, :K] = Q_als.to(torch.float32)
+    Qa = Q_als.to(dev, dt)
+    X[:, :K] = Qa.to(torch.float32)
     I_p = P @ torch.linalg.pinv(P.T @ P)
     Z2 = U2 * S2
     X[:, 16:16 + K] = (0.5 * (V2 @ (Z2.T @ I_p)) + f.reshape(1, -1) @ I_p).to(torch.float32)
+    X[:, 32:32 + K] = (torch.log(Qa.clamp_min(1e-5)) / math.log(1e-5)).to(torch.float32)
     s = V2 * S2
     e = (s * s).sum(dim=1)
     tail = (s[:, K:] * s[:, K:]).sum(dim=1) if s.shape[1] > K else torch.zeros_like(e)
-    X[:, 32] = (tail / e.clamp_min(1e-30)).to(torch.float32)
+    X[:, 48] = (tail / e.clamp_min(1e-30)).to(torch.float32)
     nrm = e.sqrt()
-    X[:, 33] = torch.log(nrm.clamp_min(1e-30) / nrm.median().clamp_min(1e-30)).to(torch.float32)
-    X[:, 34] = K / 16.0
-    X[:, 35] = 1.0 / K
-    prof = (S2[1:17] / S2[0]).to(torch.float32)
-    X[:, 36:36 + prof.numel()] = prof
+    X[:, 49] = torch.log(nrm.clamp_min(1e-30) / nrm.median().clamp_min(1e-30)).to(torch.float32)
+    X[:, 50] = K / 16.0
+    X[:, 51] = 1.0 / K
+    prof = (S2[1:13] / S2[0]).to(torch.float32)
+    X[:, 52:52 + prof.numel()] = prof
     return X
 
 
@@ -119,6 +124,23 @@ def p_from_q(U2: torch.Tensor, S2: torch.Tensor, V2: torch.Tensor, f: torch.Tens
     A_q = Q.T @ Q
     I_q = Q @ torch.linalg.pinv(A_q)
     P_free = 0.5 * (Z2 @ (V2.T @ I_q)) + torch.outer(f, I_q.sum(dim=0))
+    P = batch_bvls_bpp(A_q, P_free @ A_q, 0.0, 1.0)
+    mapP(P)
+    return P
+
+
+def p_from_q_data(G: PackedGenotypes, f: torch.Tensor, Q: torch.Tensor) -> torch.Tensor:
+    """The same P-update with the exact centred genotypes in place of the
+    rank-2K SVD: one decode-fused pass X (Q pinv(Q^T Q)) (adx_svd_xw)."""
+    from .als_gpu import mapP
+    from .solvers import batch_bvls_bpp
+    from .svd_gpu import xw
+    dt = torch.float64
+    Q = Q.to(dt)
+    A_q = Q.T @ Q
+    I_q = Q @ torch.linalg.pinv(A_q)
+    Y = xw(G, f.to(torch.float32).contiguous(), I_q.to(torch.float32).contiguous()).to(dt)
+    P_free = 0.5 * Y + torch.outer(f.to(dt), I_q.sum(dim=0))
     P = batch_bvls_bpp(A_q, P_free @ A_q, 0.0, 1.0)
     mapP(P)
     return P

