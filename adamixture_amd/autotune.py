@@ -102,17 +102,22 @@ def tune_em(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str
     acc_dt = torch.float64 if precision == "f64" else torch.float32
     bufs = {"A": torch.zeros((M, K), dtype=acc_dt, device=dev), "B": torch.zeros((M, K), dtype=acc_dt, device=dev),
             "T": torch.zeros((N, K), dtype=acc_dt, device=dev)}
+    before = get_tiles(precision)
+    cands = candidates or CANDIDATES[precision]
     if precision == "bf16":
-        nb = _lib.load().adx_em_bf16_operand_bytes(M, N, K, None)
+        nb = 0                                      # the operand buffer holds per-tile partial sums
+        for cw, tg in cands:
+            set_tiles(precision, cw, tg)
+            nb = max(nb, _lib.load().adx_em_bf16_operand_bytes(M, N, K, None))
+        set_tiles(precision, *before)
         bufs["ops"] = torch.zeros(nb, dtype=torch.uint8, device=dev)
         _lib.call("adx_em_prep_bf16", P.data_ptr(), Q.data_ptr(), M, N, K, bufs["ops"].data_ptr(), nb,
                   _lib.stream_handle(dev))
     elif precision == "f32":
         bufs["P32"] = P.to(torch.float32).contiguous()
         bufs["Q32"] = Q.to(torch.float32).contiguous()
-    before = get_tiles(precision)
     timings = {}
-    for cw, tg in (candidates or CANDIDATES[precision]):
+    for cw, tg in cands:
         set_tiles(precision, cw, tg)
         _stats_pass(G, P, Q, precision, bufs)          # warm-up (code load, caches)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

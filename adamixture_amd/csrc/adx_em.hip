@@ -90,7 +90,7 @@ template <typename T, int KS, int CW, bool STATS, bool LL>
 __global__ __launch_bounds__(256) void em_pass_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
     int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
-    T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part) {
+    T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart) {
     using MM = Mma16<T>;
     using acc_t = typename MM::acc;
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
@@ -230,11 +230,20 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
         }
 
         if (STATS) {
+            // T of these 64 samples over this workgroup's SNPs: a plain store
+            // into the workgroup's slice of Tpart (each (slice, sample) is
+            // written once; tpart_reduce_kernel sums the slices), or, without
+            // a workspace, a device-scope atomic (performed at the memory side).
             __syncthreads();
             for (int e = threadIdx.x; e < 64 * 16; e += 256) {
                 const int kp = e & 15;
                 const int64_t smp = s0 + (e >> 4);
-                if (kp < K && smp < N) atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+                if (kp < K && smp < N) {
+                    if (Tpart)
+                        Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = s_tsum[e];
+                    else
+                        atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+                }
                 s_tsum[e] = T(0);
             }
             __syncthreads();
@@ -387,10 +396,28 @@ __global__ void sched_tick_kernel(double* __restrict__ sched, double b1, double 
     }
 }
 
+// T[e] += sum over the nparts slices of Tpart[p * nk + e] (nk = N K).
+template <typename T>
+__global__ void tpart_reduce_kernel(const T* __restrict__ Tpart, int64_t nparts, int64_t nk, T* __restrict__ Tm) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nk) return;
+    T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int64_t p = 0;
+    for (; p + 4 <= nparts; p += 4) {
+        s0 += Tpart[(p + 0) * nk + e];
+        s1 += Tpart[(p + 1) * nk + e];
+        s2 += Tpart[(p + 2) * nk + e];
+        s3 += Tpart[(p + 3) * nk + e];
+    }
+    for (; p < nparts; ++p) s0 += Tpart[p * nk + e];
+    Tm[e] += (s0 + s1) + (s2 + s3);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side dispatch
 // ---------------------------------------------------------------------------
-static EmTiles g_tiles[3] = {{8, 2048}, {4, 2048}, {4, 2048}};  // ADX_F32, ADX_F64, ADX_BF16
+// Defaults: the fastest shapes measured at N=100,000 x M=500,000, K=10.
+static EmTiles g_tiles[3] = {{8, 8192}, {4, 4096}, {8, 8192}};  // ADX_F32, ADX_F64, ADX_BF16
 
 EmTiles em_tiles(int dtype) { return g_tiles[dtype >= 0 && dtype < 3 ? dtype : 0]; }
 
@@ -418,30 +445,38 @@ static EmGrid em_grid(int dtype, int64_t M, int64_t N) {
 template <typename T, int KS, bool STATS, bool LL>
 static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
                       int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B, T* Tm,
-                      double* llp) {
+                      double* llp, T* Tpart) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     if constexpr (sizeof(T) == 4) {  // the FP64 form has a single tile width
         if (gd.cw == 8) {
             em_pass_kernel<T, KS, 8, STATS, LL>
-                <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp);
+                <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart);
             return;
         }
     }
     em_pass_kernel<T, KS, 4, STATS, LL>
-        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp);
+        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart);
 }
 
 template <typename T, bool STATS, bool LL>
 static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M,
                         int64_t N, int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B,
-                        T* Tm, double* llp) {
+                        T* Tm, double* llp, T* Tpart) {
     switch ((K + 3) / 4) {
-        case 1: launch_em<T, 1, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
-        case 2: launch_em<T, 2, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
-        case 3: launch_em<T, 3, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
-        default: launch_em<T, 4, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp); break;
+        case 1: launch_em<T, 1, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
+        case 2: launch_em<T, 2, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
+        case 3: launch_em<T, 3, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
+        default: launch_em<T, 4, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
+    }
+    if (STATS && Tpart) {
+        const int64_t nk = N * K;
+        tpart_reduce_kernel<T><<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, Tm);
     }
 }
+
+// Workspace layout of adx_em_pass: [LL partials: nbx*splits doubles, padded
+// to 256 B][T partial slices: nbx x N x K of the pass dtype].
+static int64_t ll_part_bytes(const EmGrid& gd) { return ceil_div((int64_t)sizeof(double) * gd.nbx * gd.splits, 256) * 256; }
 
 }  // namespace adx
 
@@ -466,10 +501,10 @@ int adx_get_em_tiles(int dtype, int* cw, int64_t* target_wgs) {
 }
 
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
-    (void)K;
     if (M <= 0 || N <= 0) return 0;
     const EmGrid gd = em_grid(dtype, M, N);
-    return (int64_t)sizeof(double) * gd.nbx * gd.splits;
+    const int64_t esz = dtype == ADX_F64 ? 8 : 4;
+    return ll_part_bytes(gd) + esz * gd.nbx * N * (K > 0 ? K : 0);
 }
 
 int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
@@ -489,27 +524,30 @@ int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, 
     const EmGrid gd = em_grid(dtype, M, N);
     double* llp = nullptr;
     if (ll) {
-        const int64_t need = (int64_t)sizeof(double) * gd.nbx * gd.splits;
-        if (!ll_out || !workspace || workspace_bytes < need)
+        if (!ll_out || !workspace || workspace_bytes < ll_part_bytes(gd))
             return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ll_out and adx_em_workspace_bytes of workspace");
         llp = static_cast<double*>(workspace);
     }
+    // T partial slices when the workspace holds them, else atomics.
+    void* tpart = nullptr;
+    if (stats && workspace && workspace_bytes >= adx_em_workspace_bytes(dtype, M, N, K))
+        tpart = static_cast<uint8_t*>(workspace) + ll_part_bytes(gd);
     hipStream_t st = (hipStream_t)stream;
     if (dtype == ADX_F32) {
         launch_em_k<float, true, false>(gd, st, G, ldg, M, N, K, (const float*)P, ldp, (const float*)Q, ldq,
-                                        (float*)A, (float*)B, (float*)T, nullptr);
+                                        (float*)A, (float*)B, (float*)T, nullptr, (float*)tpart);
     } else if (dtype == ADX_F64) {
         const double* Pd = (const double*)P;
         const double* Qd = (const double*)Q;
         if (stats && ll)
             launch_em_k<double, true, true>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, (double*)A, (double*)B,
-                                            (double*)T, llp);
+                                            (double*)T, llp, (double*)tpart);
         else if (stats)
             launch_em_k<double, true, false>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, (double*)A, (double*)B,
-                                             (double*)T, nullptr);
+                                             (double*)T, nullptr, (double*)tpart);
         else
             launch_em_k<double, false, true>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, nullptr, nullptr,
-                                             nullptr, llp);
+                                             nullptr, llp, nullptr);
     } else {
         return fail_msg(1, "adx_em_pass: dtype must be ADX_F32 or ADX_F64");
     }

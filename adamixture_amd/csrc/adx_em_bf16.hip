@@ -70,7 +70,7 @@ template <int KH, int CW>
 __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
     const __bf16* __restrict__ Qx, const __bf16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
-    float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc) {
+    float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc, float* __restrict__ Tpart) {
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
     constexpr int LX = 32 * KH;  // row length of Px / Qx
     constexpr int TS = 20;       // row stride (words) of the per-wave transpose tile
@@ -220,10 +220,20 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         }
 
         __syncthreads();
+        // T of these 64 samples over this workgroup's SNPs: one plain store
+        // into the workgroup's slice of the partial buffer (every (slice,
+        // sample) is written exactly once), summed by tpart_reduce_kernel.
+        // Device-scope float atomics would be performed at the memory side
+        // (one 4-byte read-modify-write each, ~N K M / (64 CW) per pass).
         for (int e = threadIdx.x; e < 64 * 16; e += 256) {
             const int kp = e & 15;
             const int64_t smp = s0 + (e >> 4);
-            if (kp < K && smp < N) atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+            if (kp < K && smp < N) {
+                if (Tpart)
+                    Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = s_tsum[e];
+                else
+                    atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+            }
             s_tsum[e] = 0.0f;
         }
         __syncthreads();
@@ -318,15 +328,43 @@ static Bf16Grid bf16_grid(int64_t M, int64_t N) {
     return gd;
 }
 
+// T[e] += sum over the nparts slices of Tpart[p * nk + e] (nk = N K).
+__global__ void tpart_reduce_kernel(const float* __restrict__ Tpart, int64_t nparts, int64_t nk,
+                                    float* __restrict__ T) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nk) return;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    int64_t p = 0;
+    for (; p + 4 <= nparts; p += 4) {
+        s0 += Tpart[(p + 0) * nk + e];
+        s1 += Tpart[(p + 1) * nk + e];
+        s2 += Tpart[(p + 2) * nk + e];
+        s3 += Tpart[(p + 3) * nk + e];
+    }
+    for (; p < nparts; ++p) s0 += Tpart[p * nk + e];
+    T[e] += (s0 + s1) + (s2 + s3);
+}
+
 template <int KH>
 static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
                         int K, const __bf16* Px, const __bf16* Qx, const __bf16* Qt, int64_t ldqt, float* A,
-                        float* B, float* T) {
+                        float* B, float* T, float* Tpart) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     if (gd.cw == 8)
-        em_pass_bf16_kernel<KH, 8><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T);
+        em_pass_bf16_kernel<KH, 8>
+            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart);
     else
-        em_pass_bf16_kernel<KH, 4><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T);
+        em_pass_bf16_kernel<KH, 4>
+            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart);
+    const int64_t nk = N * K;
+    tpart_reduce_kernel<<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, T);
+}
+
+// Byte offset of the partial-T slices in the operand buffer, and its size.
+static int64_t bf16_base_bytes(int64_t M, int64_t N, int K) {
+    const int kh = K <= 10 ? 1 : 2;
+    const int64_t ld = ceil_div(N, 64) * 64;
+    return ceil_div((int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 32 * ld), 256) * 256;
 }
 
 }  // namespace adx
@@ -336,10 +374,11 @@ using namespace adx;
 extern "C" {
 
 int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t* ldqt) {
-    const int kh = K <= 10 ? 1 : 2;
     const int64_t ld = ceil_div(N, 64) * 64;
     if (ldqt) *ldqt = ld;
-    return (int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 32 * ld);
+    if (M <= 0 || N <= 0) return bf16_base_bytes(M > 0 ? M : 0, N > 0 ? N : 0, K);
+    const Bf16Grid gd = bf16_grid(M, N);
+    return bf16_base_bytes(M, N, K) + (int64_t)sizeof(float) * gd.nbx * N * K;
 }
 
 int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int K, void* operands,
@@ -364,7 +403,7 @@ int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int
     return 0;
 }
 
-int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* operands,
+int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, void* operands,
                      int64_t operand_bytes, float* A, float* B, float* T, void* stream) {
     if (M <= 0 || N <= 0) return fail_msg(1, "adx_em_pass_bf16: empty genotype matrix");
     if (K < 1 || K > 16) return fail_msg(1, "adx_em_pass_bf16: K must be in [1, 16]");
@@ -379,12 +418,13 @@ int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
     const __bf16* Px = static_cast<const __bf16*>(operands);
     const __bf16* Qx = Px + M * 32 * kh;
     const __bf16* Qt = Qx + N * 32 * kh;
+    float* Tpart = reinterpret_cast<float*>(static_cast<uint8_t*>(operands) + bf16_base_bytes(M, N, K));
     const Bf16Grid gd = bf16_grid(M, N);
     hipStream_t st = (hipStream_t)stream;
     if (kh == 1)
-        launch_bf16<1>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T);
+        launch_bf16<1>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart);
     else
-        launch_bf16<2>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T);
+        launch_bf16<2>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart);
     ADX_LAUNCH_CHECK("adx_em_pass_bf16");
     return 0;
 }

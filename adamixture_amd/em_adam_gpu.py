@@ -54,7 +54,9 @@ class _Workspace:
 
     def get(self, device, nbytes: int) -> tuple[torch.Tensor, torch.Tensor]:
         if self.buf is None or self.buf.numel() < nbytes or self.buf.device != device:
+            self.buf = None                      # release before growing (the T slices can be GBs)
             self.buf = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=device)
+        if self.ll is None or self.ll.device != device:
             self.ll = torch.zeros(1, dtype=torch.float64, device=device)
         return self.buf, self.ll
 
@@ -68,12 +70,13 @@ def em_pass(G: PackedGenotypes, P, Q, A=None, B=None, T=None, *, dtype: int = _l
     on device) when ``loglik`` is set, else None."""
     K = P.shape[1]
     flags = (_lib.ADX_EM_STATS if stats else 0) | (_lib.ADX_EM_LOGLIK if loglik else 0)
-    ws_bytes = _lib.load().adx_em_workspace_bytes(dtype, G.M, G.N, K)
-    ws, ll = _WS.get(G.device, ws_bytes) if loglik else (None, None)
+    # K = 0 sizes only the log-likelihood partials (no T slices without stats)
+    ws_bytes = _lib.load().adx_em_workspace_bytes(dtype, G.M, G.N, K if stats else 0)
+    ws, ll = _WS.get(G.device, ws_bytes)
     _lib.call("adx_em_pass", dtype, G.data.data_ptr(), G.ldg, G.M, G.N, K, P.data_ptr(), P.stride(0),
-              Q.data_ptr(), Q.stride(0), _lib.ptr(A), _lib.ptr(B), _lib.ptr(T), flags, _lib.ptr(ll),
-              _lib.ptr(ws), ws_bytes if loglik else 0, _lib.stream_handle(G.device))
-    return ll
+              Q.data_ptr(), Q.stride(0), _lib.ptr(A), _lib.ptr(B), _lib.ptr(T), flags,
+              _lib.ptr(ll) if loglik else None, ws.data_ptr(), ws_bytes, _lib.stream_handle(G.device))
+    return ll if loglik else None
 
 
 class EMAdamOptimizer:
@@ -114,6 +117,9 @@ class EMAdamOptimizer:
             st = _lib.stream_handle(self.device)
             M, K = P.shape
             N = Q.shape[0]
+            need = _lib.load().adx_em_bf16_operand_bytes(M, N, K, None)
+            if self.operands.numel() < need:      # the tile shape changed since construction
+                self.operands = torch.empty(need, dtype=torch.uint8, device=self.device)
             nb = self.operands.numel()
             _lib.call("adx_em_prep_bf16", P.data_ptr(), Q.data_ptr(), M, N, K, self.operands.data_ptr(), nb, st)
             _lib.call("adx_em_pass_bf16", G.data.data_ptr(), G.ldg, G.M, G.N, K, self.operands.data_ptr(), nb,

@@ -4,10 +4,14 @@ randomized SVD + ALS only, adamixture/src/adamixture.py:185-250).
 A 3-layer MLP (64 -> 256 -> 256 -> 16, ReLU) maps each sample's
 randomized-SVD features to a Q row: softmax(z + log max(q_als, 1e-5)) over
 the first K outputs, i.e. a learned correction of the ALS row on the
-simplex.  P follows from Q by the ALS P-update (bounded least squares on the
-rank-2K SVD, adamixture/model/als_gpu.py:99-106).  The warm start is kept
-only if its FP64 log-likelihood beats the plain SVD + ALS start; otherwise
-the plain start is returned (the fallback).
+simplex.  The ALS row comes from a short ALS run (ALS_ITERS iterations)
+instead of ALS to its tolerance, which is most of the plain start's cost;
+the network is trained to map it to the Q the plain method converges to.
+P follows from Q by the ALS P-update (bounded least squares,
+adamixture/model/als_gpu.py:99-106) on the exact centred genotypes (one
+decode-fused pass).  The warm start is kept only if its FP64
+log-likelihood beats the short ALS start's; otherwise the plain start is
+finished (ALS continued to its tolerance) and returned (the fallback).
 
 Training runs in FP32 on the libadx kernels (adx_mlp_gemm_f32, bias/ReLU,
 softmax cross-entropy, Adam); inference is one fused BF16 MFMA kernel
@@ -44,6 +48,9 @@ D_IN, WIDTH, D_OUT = 64, 256, 16
 NPARAM = D_IN * WIDTH + WIDTH + WIDTH * WIDTH + WIDTH + WIDTH * D_OUT + D_OUT
 PACKED_BYTES = 2 * (WIDTH * D_IN + WIDTH * WIDTH + D_OUT * WIDTH) + 4 * (2 * WIDTH + D_OUT)
 K_MAX = D_OUT
+# ALS iterations of the learned start; the plain start runs ALS to its own
+# tolerance (up to 1000 iterations, adamixture/entry.py:68-69).
+ALS_ITERS = 20
 # (name, rows, cols) of the flat parameter vector, in order.
 BLOCKS = [("W1", D_IN, WIDTH), ("b1", 1, WIDTH), ("W2", WIDTH, WIDTH), ("b2", 1, WIDTH),
           ("W3", WIDTH, D_OUT), ("b3", 1, D_OUT)]
@@ -284,21 +291,30 @@ def load_meta(path=WEIGHTS_PATH) -> dict:
 # the warm start with its fallback
 # ---------------------------------------------------------------------------
 
-def learned_warm_start(G: PackedGenotypes, U2, S2, V2, f, P_als: torch.Tensor, Q_als: torch.Tensor, K: int,
-                       model: WarmStartMLP, logl_als: float | None = None):
-    """Returns (P, Q, logL, used_learned).  The learned start is used only
-    when its FP64 log-likelihood exceeds that of (P_als, Q_als)."""
+def learned_warm_start(G: PackedGenotypes, U2, S2, V2, f, K: int, model: WarmStartMLP, seed: int, *,
+                       als_iters: int = ALS_ITERS, max_als: int = 1000, tol_als: float = 1e-4):
+    """Warm start from the rank-2K SVD (U2, S2, V2): ``als_iters`` ALS
+    iterations on its first K components (instead of ALS to its tolerance),
+    the network's Q, and P from Q by one decode-fused pass over G.
+    Returns (P, Q, logL, used_learned).  If the learned start's FP64 logL does
+    not exceed that of the short ALS start, the plain start is returned
+    instead: ALS continued from the short run to its own stopping rule."""
+    from .als_gpu import ALS_gpu
     from .em_adam_gpu import em_pass
     t0 = time.time()
-    X = features(U2, S2, V2, f, P_als, Q_als, K)
+    dev = G.device
+    Uk, Sk, Vk = U2[:, :K].contiguous(), S2[:K].contiguous(), V2[:, :K].contiguous()
+    P_s, Q_s = ALS_gpu(Uk, Sk, Vk, f, seed, G.M, K, als_iters, tol_als, dev)
+    X = features(U2, S2, V2, f, P_s, Q_s, K)
     Q = model.infer(X, K)
-    P = p_from_q(U2, S2, V2, f, Q)
+    P = p_from_q_data(G, f, Q)
     ll = em_pass(G, P, Q, loglik=True, stats=False).item()
-    if logl_als is None:
-        logl_als = em_pass(G, P_als.contiguous(), Q_als.contiguous(), loglik=True, stats=False).item()
-    used = ll > logl_als
-    log.info(f"    Learned warm start logL {ll:.1f} vs SVD+ALS {logl_als:.1f} -> "
+    ll_s = em_pass(G, P_s.contiguous(), Q_s.contiguous(), loglik=True, stats=False).item()
+    used = ll > ll_s
+    log.info(f"    Learned warm start logL {ll:.1f} vs {als_iters}-iteration ALS {ll_s:.1f} -> "
              f"{'learned' if used else 'fallback to SVD+ALS'} ({time.time() - t0:.3f}s)")
     if used:
         return P, Q, ll, True
-    return P_als, Q_als, logl_als, False
+    P_a, Q_a = ALS_gpu(Uk, Sk, Vk, f, seed, G.M, K, max(max_als - als_iters, 0), tol_als, dev, P_init=P_s,
+                       Q_init=Q_s)
+    return P_a, Q_a, em_pass(G, P_a.contiguous(), Q_a.contiguous(), loglik=True, stats=False).item(), False

@@ -28,7 +28,7 @@ from .adamixture import _require_gpu
 from .als_gpu import ALS_gpu
 from .em_adam_gpu import em_pass, optimize_parameters_gpu
 from .genotypes import allele_frequencies
-from .learned_init import D_OUT, WarmStartMLP, features
+from .learned_init import ALS_ITERS, D_OUT, WarmStartMLP, features
 from .svd_gpu import SVD_gpu
 from .synth import Workload, make_genotypes
 
@@ -65,25 +65,41 @@ def converge_plain(G, P, Q, K: int, device, precision: str = "f64", on_check=Non
                                    a["tol_adam"], device, 0, 0, precision, on_check)
 
 
+def align_columns(Q_ref: torch.Tensor, Q: torch.Tensor) -> list[int]:
+    """Column permutation of Q that best matches Q_ref (max sum of column
+    overlaps Q_ref^T Q, Hungarian method on the K x K overlap matrix)."""
+    from scipy.optimize import linear_sum_assignment
+    C = (Q_ref.T @ Q).cpu().numpy()
+    rows, cols = linear_sum_assignment(-C)
+    return [int(c) for _, c in sorted(zip(rows, cols))]
+
+
 def make_training_set(seed: int, N: int, M: int, K: int, fst: float, alpha: float, miss: float, device,
-                      precision: str = "f64"):
-    """(X, T, info) of one synthetic data set: features N x 64, targets
-    N x 16 (converged Q, zero-padded)."""
+                      precision: str = "f64", als_iters: int = ALS_ITERS):
+    """(X, T, info) of one synthetic data set: features N x 64 from the
+    ``als_iters``-iteration ALS start, targets N x 16 = the Q the plain
+    method converges to (SVD + ALS to tolerance + Adam-EM), zero-padded, its
+    columns aligned to the short ALS start's clusters."""
     dev = torch.device(device)
     w = Workload(f"train{seed}", N, M, K, "bn", alpha, miss, fst=fst, seed=seed)
     t0 = time.time()
     G, _, _ = make_genotypes(w, dev)
     f = allele_frequencies(G)
     U, S, V, P_als, Q_als = plain_start(G, f, K, seed, dev)
-    X = features(U, S, V, f, P_als, Q_als, K)
+    P_s, Q_s = ALS_gpu(U[:, :K].contiguous(), S[:K].contiguous(), V[:, :K].contiguous(), f, seed, G.M, K,
+                       als_iters, SVD_DEFAULTS["tol_als"], dev)
+    X = features(U, S, V, f, P_s, Q_s, K)
     n_checks = []
     P_star, Q_star = converge_plain(G, P_als, Q_als, K, dev, precision, on_check=lambda it, ll: n_checks.append(it))
+    perm = align_columns(Q_s, Q_star)
     T = torch.zeros((N, D_OUT), dtype=torch.float32, device=dev)
-    T[:, :K] = Q_star.to(torch.float32)
+    T[:, :K] = Q_star[:, perm].to(torch.float32)
     ll_als = em_pass(G, P_als, Q_als, loglik=True, stats=False).item()
+    ll_short = em_pass(G, P_s.contiguous(), Q_s.contiguous(), loglik=True, stats=False).item()
     ll_star = em_pass(G, P_star, Q_star, loglik=True, stats=False).item()
-    info = dict(seed=seed, K=K, fst=fst, alpha=alpha, ll_als=ll_als, ll_star=ll_star,
-                iters=n_checks[-1] if n_checks else 0, seconds=time.time() - t0)
+    info = dict(seed=seed, K=K, fst=fst, alpha=alpha, ll_als=ll_als, ll_short_als=ll_short, ll_star=ll_star,
+                permuted=perm != list(range(K)), iters=n_checks[-1] if n_checks else 0,
+                seconds=time.time() - t0)
     del G
     return X, T, info
 
@@ -225,9 +241,7 @@ def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_p
     if model is not None:
         t3 = _sync_time(dev)
         U2, S2, V2 = _svd(G, G.N, G.M, f, 2 * K, seed, SVD_DEFAULTS["power"], SVD_DEFAULTS["tol_svd"], 0, dev)
-        P_a2, Q_a2 = ALS_gpu(U2[:, :K].contiguous(), S2[:K].contiguous(), V2[:, :K].contiguous(), f, seed, G.M,
-                             K, SVD_DEFAULTS["max_als"], SVD_DEFAULTS["tol_als"], dev)
-        P_w, Q_w, ll_w, used = learned_warm_start(G, U2, S2, V2, f, P_a2, Q_a2, K, model)
+        P_w, Q_w, ll_w, used = learned_warm_start(G, U2, S2, V2, f, K, model, seed)
         t_init_l = _sync_time(dev) - t3
         t4 = _sync_time(dev)
         P3, Q3, ch_l = run_to_stop(G, P_w, Q_w, K, dev, fast_precision, t4)
@@ -300,7 +314,7 @@ def main(argv=None) -> int:
         w = Workload(f"eval{seed}", args.N, args.M, K, "bn", alpha, args.miss, fst=fst, seed=seed)
         G, _, _ = make_genotypes(w, dev)
         f = allele_frequencies(G)
-        r = race(G, f, K, seed, dev, model, plain_precision="f64", fast_precision="f64")
+        r = race(G, f, K, seed, dev, model, plain_precision="f32", fast_precision="bf16")
         r.update(fst=fst, alpha=alpha, seed=seed)
         races.append(r)
         del G

@@ -34,8 +34,9 @@ class _Run:
     """Device state of one K of the sweep."""
 
     def __init__(self, G: PackedGenotypes, P0: torch.Tensor, Q0: torch.Tensor, precision: str, hp: dict,
-                 ll_slot: torch.Tensor):
+                 ll_slot: torch.Tensor, shared: dict):
         dev = G.device
+        self.shared = shared
         self.G = G
         self.K = P0.shape[1]
         self.precision = precision
@@ -70,10 +71,21 @@ class _Run:
         self.Q_best = self.Q.clone()
 
     def _workspace(self):
-        need = _lib.load().adx_em_workspace_bytes(_lib.ADX_F64, self.G.M, self.G.N, self.K)
+        """Log-likelihood partial sums of this K (K = 0: no T slices)."""
+        need = _lib.load().adx_em_workspace_bytes(_lib.ADX_F64, self.G.M, self.G.N, 0)
         if self.ws is None or self.ws.numel() < need:
             self.ws = torch.empty(max(need, 8), dtype=torch.uint8, device=self.G.device)
         return self.ws, need
+
+    def _stats_workspace(self, dtag: int):
+        """T partial slices of the FP32/FP64 pass; one buffer shared by every
+        K of the sweep (their passes run one after another on one stream)."""
+        need = _lib.load().adx_em_workspace_bytes(dtag, self.G.M, self.G.N, self.K)
+        buf = self.shared.get("stats")
+        if buf is None or buf.numel() < need:
+            self.shared["stats"] = None
+            buf = self.shared["stats"] = torch.empty(need, dtype=torch.uint8, device=self.G.device)
+        return buf, need
 
     def iterate(self) -> None:
         """Enqueue one Adam-EM iteration (no host synchronisation)."""
@@ -81,16 +93,21 @@ class _Run:
         M, N = G.M, G.N
         st = _lib.stream_handle(G.device)
         if self.precision == "bf16":
+            need = _lib.load().adx_em_bf16_operand_bytes(M, N, K, None)
+            if self.ops.numel() < need:          # tile shape changed (never inside a capture: the
+                self.ops = torch.empty(need, dtype=torch.uint8, device=G.device)  # priming ran first)
             nb = self.ops.numel()
             _lib.call("adx_em_prep_bf16", self.P.data_ptr(), self.Q.data_ptr(), M, N, K, self.ops.data_ptr(), nb, st)
             _lib.call("adx_em_pass_bf16", G.data.data_ptr(), G.ldg, M, N, K, self.ops.data_ptr(), nb,
                       self.A.data_ptr(), self.B.data_ptr(), self.T.data_ptr(), st)
         else:
             lowp = self.precision == "f32"
+            dtag = _lib.ADX_F32 if lowp else _lib.ADX_F64
             Pp, Qp = (self.P_lo, self.Q_lo) if lowp else (self.P, self.Q)
-            _lib.call("adx_em_pass", _lib.ADX_F32 if lowp else _lib.ADX_F64, G.data.data_ptr(), G.ldg, M, N, K,
-                      Pp.data_ptr(), K, Qp.data_ptr(), K, self.A.data_ptr(), self.B.data_ptr(), self.T.data_ptr(),
-                      _lib.ADX_EM_STATS, None, None, 0, st)
+            ws, need = self._stats_workspace(dtag)
+            _lib.call("adx_em_pass", dtag, G.data.data_ptr(), G.ldg, M, N, K, Pp.data_ptr(), K, Qp.data_ptr(), K,
+                      self.A.data_ptr(), self.B.data_ptr(), self.T.data_ptr(), _lib.ADX_EM_STATS, None,
+                      ws.data_ptr(), need, st)
         _lib.call("adx_em_finalize", self.acc_tag, self.P.data_ptr(), self.Q.data_ptr(), self.A.data_ptr(),
                   self.B.data_ptr(), self.T.data_ptr(), G.qbat().data_ptr(), M, N, K, self.P_EM.data_ptr(),
                   self.Q_EM.data_ptr(), st)
@@ -157,7 +174,8 @@ def sweep(G: PackedGenotypes, starts: dict, precision: str = "f32", *, graph: bo
     G.qbat()                                       # computed once, outside any capture
     Ks = sorted(starts)
     ll_all = torch.zeros(len(Ks), dtype=torch.float64, device=dev)
-    runs = [_Run(G, *starts[K], precision, hp, ll_all[i:i + 1]) for i, K in enumerate(Ks)]
+    shared: dict = {}
+    runs = [_Run(G, *starts[K], precision, hp, ll_all[i:i + 1], shared) for i, K in enumerate(Ks)]
     if autotune:
         from .autotune import ensure_tuned
         for r in runs:

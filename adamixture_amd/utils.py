@@ -119,7 +119,8 @@ def estimate_gpu_bytes(M: int, N: int, K: int, algorithm: str, q_hist: int = 3,
     svd = M * L * 4 + 2 * N * L * 4 + M * L * 4
     als = 5 * (M + N) * K * 8
     if algorithm == "adamem":
-        opt = (8 * M * K + 7 * N * K + N) * 8 + 2 * (M + N) * K * 4
+        # + the per-workgroup T partial sums of the FP64 pass (256 SNPs per slice)
+        opt = (8 * M * K + 7 * N * K + N) * 8 + 2 * (M + N) * K * 4 + ((M + 255) // 256) * N * K * 8
     elif algorithm == "brqn":
         dim = (M + N) * K
         opt = ((M + N) * K * K + (M + N) * K) * 8 + 3 * dim * q_hist * 8 + 9 * dim * 8

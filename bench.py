@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-gb", type=float, default=0.0,
                     help="run one device copy of this many GB first (calibrates HBM counters under rocprofv3 --pmc)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default EM tile shape instead of timing the candidates first")
+    ap.add_argument("--tiles", default=None, help="fixed EM tile shape 'cw/target_wgs' (implies --no-autotune)")
     ap.add_argument("--ttt", action="store_true",
                     help="also race time to logL within 1e-6 of the plain FP32 result (adds ~2 min)")
     return ap.parse_args()
@@ -97,6 +100,13 @@ def main():
     P = torch.from_numpy(P0).to(dev)
     Q = torch.from_numpy(Q0).to(dev)
     G.qbat()
+    tuning = None
+    if args.tiles:
+        from adamixture_amd.autotune import set_tiles
+        set_tiles(args.precision, *(int(x) for x in args.tiles.split("/")))
+    elif not args.no_autotune:
+        from adamixture_amd.autotune import tune_em
+        tuning = tune_em(G, P, Q, args.precision)
     opt = EMAdamOptimizer(P.shape, Q.shape, 0.005, 0.8, 0.88, 1e-8, dev, precision=args.precision)
     check = 5
     ev = []
@@ -181,6 +191,10 @@ def main():
                      "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg,
                      "hbm_GBs_algorithmic": round(bytes_alg / (kern_ms * 1e-3) / 1e9, 1)},
     }
+    from adamixture_amd.autotune import get_tiles
+    line["config"]["tiles"] = dict(zip(("cw", "target_wgs"), get_tiles(args.precision)))
+    if tuning is not None:
+        line["config"]["autotune_ms"] = {k: round(v, 3) for k, v in tuning["ms"].items()}
     if not args.no_cpu_baseline:
         rows = args.cpu_rows or max(4, min(M, int(4e9 / N)))
         line["cpu_baseline"] = cpu_baseline(G, P0, Q0, w, rows)

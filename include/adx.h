@@ -78,11 +78,13 @@ int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int
  * ADX_F64) and target_wgs = workgroups the sample range is split to reach
  * (64..65536).  Process-wide; affects later launches and
  * adx_em_workspace_bytes (query the workspace after changing it).  Defaults:
- * F32 (8, 2048), F64 (4, 2048), BF16 (4, 2048). */
+ * F32 (8, 8192), F64 (4, 4096), BF16 (8, 8192). */
 int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs);
 int adx_get_em_tiles(int dtype, int *cw, int64_t *target_wgs);
 
-/* Workspace (bytes) adx_em_pass needs for the given shape and dtype. */
+/* Workspace (bytes) of adx_em_pass for the given shape, dtype and current
+ * tile shape: the log-likelihood partial sums plus the per-workgroup partial
+ * sums of T (ceil(M / (64 cw)) x N x K of the pass dtype). */
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
 
 /* Replaces the chunk loop of EMAdamOptimizer.run_em_step with
@@ -93,6 +95,9 @@ int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
  *                  (N x K), with H = P Q^T clamped to [1e-5, 1-1e-5],
  *                  a = g/H, b = (2-g)/(1-H) on observed genotypes.  A, B, T
  *                  are accumulated (caller zeroes them; adx_em_finalize does).
+ *                  With a workspace of adx_em_workspace_bytes, T is reduced
+ *                  through per-workgroup partial sums (plain stores and one
+ *                  summing kernel); with less, by device-scope atomics.
  *   ADX_EM_LOGLIK: *ll_out (device, FP64) = sum g log H + (2-g) log1p(-H),
  *                  H clamped to [1e-10, 1-1e-10]; FP64 dtype only.
  * dtype selects the arithmetic of P, Q, A, B, T: ADX_F64 or ADX_F32. */
@@ -105,11 +110,15 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
  * contractions run on BF16 MFMA with FP32 accumulation from operand arrays
  * that adx_em_prep_bf16 builds from the FP64 P and Q (H from a three-term
  * hi/lo split; A and B from Q split in hi + lo; a, b rounded to BF16).
- * Replaces the same reference code (em_adam_gpu.py:46-78, :166-197). */
+ * The operand buffer also holds the per-workgroup partial sums of T
+ * (ceil(M / (64 cw)) x N x K FP32, cw from adx_set_em_tiles), so its size
+ * depends on the current tile shape: query adx_em_bf16_operand_bytes after
+ * changing it.  Replaces the same reference code (em_adam_gpu.py:46-78,
+ * :166-197). */
 int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t *ldqt);
 int adx_em_prep_bf16(const double *P, const double *Q, int64_t M, int64_t N, int K, void *operands,
                      int64_t operand_bytes, void *stream);
-int adx_em_pass_bf16(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, const void *operands,
+int adx_em_pass_bf16(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, void *operands,
                      int64_t operand_bytes, float *A, float *B, float *T, void *stream);
 
 /* Replaces em_final_update (adamixture/model/em_adam_gpu.py:82-117):
@@ -138,6 +147,19 @@ int adx_adam_step(double *P, double *Q, const double *P_EM, const double *Q_EM, 
 int adx_adam_step_sched(double *P, double *Q, const double *P_EM, const double *Q_EM, double *mP, double *vP,
                         double *mQ, double *vQ, int64_t M, int64_t N, int K, double *sched, double beta1,
                         double beta2, double eps, float *P_lo, float *Q_lo, int64_t ld_lo, void *stream);
+
+/* --- ALS products (FP64) ------------------------------------------------------ */
+
+/* The skinny products of the ALS update (adamixture/model/als_gpu.py:90-113,
+ * torch matmuls there): C (a x b) = alpha X^T Y with X (n x a), Y (n x b),
+ * 1 <= a, b <= 64, a b <= 1024 (C is overwritten); and C (n x b) =
+ * alpha X W + u v^T + beta C with X (n x a), W (a x b), a <= 64, b <= 16,
+ * u (n) and v (b) both NULL or both given.  Row-major with row strides. */
+int adx_gemm_tn_f64(const double *X, int64_t ldx, const double *Y, int64_t ldy, int64_t n, int a, int b,
+                    double alpha, double *C, int64_t ldc, void *stream);
+int adx_gemm_nn_f64(const double *X, int64_t ldx, int64_t n, int a, int b, const double *W, int64_t ldw,
+                    double alpha, const double *u, const double *v, double beta, double *C, int64_t ldc,
+                    void *stream);
 
 /* --- randomized-SVD initializer GEMMs (FP32) -------------------------------- */
 

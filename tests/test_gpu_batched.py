@@ -48,8 +48,12 @@ def test_sweep_graph_matches_eager(cuda, precision):
         Lb = np.array([L for _, L in b[K]["checks"]])
         print(f"K={K}: {len(La)} checks, max rel dL {np.max(np.abs(La - Lb) / np.abs(Lb)):.2e}", flush=True)
         np.testing.assert_allclose(La, Lb, rtol=1e-9)
-        np.testing.assert_allclose(a[K]["P"].cpu().numpy(), b[K]["P"].cpu().numpy(), atol=1e-6)
-        np.testing.assert_allclose(a[K]["Q"].cpu().numpy(), b[K]["Q"].cpu().numpy(), atol=1e-6)
+        # FP64 accumulation reproduces the eager run exactly here; the FP32
+        # accumulators of the f32/bf16 forms add in a different order from
+        # run to run, which 40 Adam steps amplify to ~1e-4..1e-3 in P
+        tol = 1e-12 if precision == "f64" else 3e-3
+        np.testing.assert_allclose(a[K]["P"].cpu().numpy(), b[K]["P"].cpu().numpy(), atol=tol)
+        np.testing.assert_allclose(a[K]["Q"].cpu().numpy(), b[K]["Q"].cpu().numpy(), atol=tol)
 
 
 def test_sweep_matches_single_k_optimizer(cuda):
@@ -79,7 +83,8 @@ def test_sweep_runs_until_each_k_stops(cuda):
     for K in starts:
         assert res[K]["iterations"] < 3000
         L = [L for _, L in res[K]["checks"]]
-        assert res[K]["logL"] == max(L)
+        # the best is replaced only by a check better by more than tol_adam
+        assert max(L) - HP["tol_adam"] <= res[K]["logL"] <= max(L)
 
 
 def test_qp_batch_graph_matches_eager_and_oracle(cuda):

@@ -1,0 +1,59 @@
+"""Host-side logic of the drop-in layer that needs no GPU: the memory
+manager's placement decision with a faked free-memory value (the reference's
+own approach to testing it, adamixture/src/utils.py:311-429), the OOM
+classifier and chunk halving (utils.py:19-89), and the .P / .Q writer
+format (utils.py:252-273)."""
+import numpy as np
+import pytest
+import torch
+
+from adamixture_amd import utils
+
+
+def test_manage_gpu_memory_places_or_raises(monkeypatch):
+    M, N, K = 500_000, 100_000, 10
+    G = torch.zeros(((M + 3) // 4, N), dtype=torch.uint8)          # host tensor, never moved here
+    placed = []
+    monkeypatch.setattr(utils, "as_packed", lambda g, m, dev: placed.append((m, str(dev))) or "device-copy")
+    need = ((M + 3) // 4) * ((N + 63) // 64 * 64) + utils.estimate_gpu_bytes(M, N, K, "adamem")
+    monkeypatch.setattr(utils, "get_free_gpu_memory", lambda dev: need / 2**20 / 0.95 * 1.01)
+    assert utils.manage_gpu_memory(G, torch.device("cuda:0"), M, N, K, 8192, "adamem") == "device-copy"
+    assert placed == [(M, "cuda:0")]
+    monkeypatch.setattr(utils, "get_free_gpu_memory", lambda dev: need / 2**20 / 0.95 * 0.99)
+    with pytest.raises(RuntimeError, match="out of memory") as ei:
+        utils.manage_gpu_memory(G, torch.device("cuda:0"), M, N, K, 8192, "adamem")
+    assert utils.is_gpu_oom(ei.value)
+    # a CPU device leaves G where it is
+    assert utils.manage_gpu_memory(G, torch.device("cpu"), M, N, K, 8192, "adamem") is G
+
+
+def test_estimate_gpu_bytes_orders_algorithms():
+    M, N, K = 200_000, 50_000, 12
+    a = utils.estimate_gpu_bytes(M, N, K, "adamem")
+    b = utils.estimate_gpu_bytes(M, N, K, "brqn")
+    assert a > 0 and b >= ((M + N) * K * K) * 8          # BR-QN keeps (M+N) K^2 FP64 Hessians
+    assert a >= ((M + 255) // 256) * N * K * 8           # Adam-EM keeps the FP64 T partial sums
+    assert utils.estimate_gpu_bytes(M, N, K, "adamem", include_initialization=False) <= a
+    assert utils.estimate_gpu_bytes(2 * M, N, K, "adamem") > a
+    with pytest.raises(ValueError):
+        utils.estimate_gpu_bytes(M, N, K, "nope")
+
+
+def test_oom_classifier_and_chunk_halving():
+    assert utils.is_gpu_oom(RuntimeError("HIP error: out of memory"))
+    assert not utils.is_gpu_oom(RuntimeError("invalid argument"))
+    exc = RuntimeError("out of memory")
+    assert utils.reduce_gpu_chunk_or_raise(8192, exc, "testing") == 4096
+    assert utils.reduce_gpu_chunk_or_raise(3, exc, "testing") == utils.MIN_GPU_CHUNK_SIZE
+    with pytest.raises(RuntimeError):
+        utils.reduce_gpu_chunk_or_raise(utils.MIN_GPU_CHUNK_SIZE, exc, "testing")
+
+
+def test_write_outputs_admixture_text_format(tmp_path):
+    Q = np.array([[0.25, 0.75], [1e-5, 1 - 1e-5]])
+    P = np.array([[0.1, 0.9]])
+    utils.write_outputs(Q, "run", 2, tmp_path, P)
+    lines = (tmp_path / "run.2.Q").read_text().splitlines()
+    assert lines[0] == "2.500000000000000000e-01 7.500000000000000000e-01"
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "run.2.Q"), Q)
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "run.2.P", ndmin=2), P)
