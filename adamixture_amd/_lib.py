@@ -54,6 +54,9 @@ SIGNATURES = {
     "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),
     "adx_adam_step_sched": (_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _dbl, _dbl, _dbl, _p, _p,
                                    _i64, _p]),
+    "adx_bed_to_packed": (_int, [_p, _i64, _i64, _i64, _p, _i64, _i64, _p]),
+    "adx_packed_allele_sum": (_int, [_p, _i64, _i64, _i64, _p, _p]),
+    "adx_flip_packed": (_int, [_p, _i64, _i64, _i64, _p]),
     "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p]),
     "adx_gemm_nn_f64": (_int, [_p, _i64, _i64, _int, _int, _p, _i64, _dbl, _p, _p, _dbl, _p, _i64, _p]),
     "adx_set_em_tiles": (_int, [_int, _int, _i64]),

@@ -4,14 +4,12 @@ randomized SVD + ALS only, adamixture/src/adamixture.py:185-250).
 A 3-layer MLP (64 -> 256 -> 256 -> 16, ReLU) maps each sample's
 randomized-SVD features to a Q row: softmax(z + log max(q_als, 1e-5)) over
 the first K outputs, i.e. a learned correction of the ALS row on the
-simplex.  The ALS row comes from a short ALS run (ALS_ITERS iterations)
-instead of ALS to its tolerance, which is most of the plain start's cost;
-the network is trained to map it to the Q the plain method converges to.
-P follows from Q by the ALS P-update (bounded least squares,
+simplex; the network is trained to map it to the Q the plain method
+converges to.  P follows from Q by the ALS P-update (bounded least squares,
 adamixture/model/als_gpu.py:99-106) on the exact centred genotypes (one
 decode-fused pass).  The warm start is kept only if its FP64
-log-likelihood beats the short ALS start's; otherwise the plain start is
-finished (ALS continued to its tolerance) and returned (the fallback).
+log-likelihood beats the ALS start's; otherwise the plain SVD + ALS start
+is returned (the fallback).
 
 Training runs in FP32 on the libadx kernels (adx_mlp_gemm_f32, bias/ReLU,
 softmax cross-entropy, Adam); inference is one fused BF16 MFMA kernel
@@ -48,9 +46,11 @@ D_IN, WIDTH, D_OUT = 64, 256, 16
 NPARAM = D_IN * WIDTH + WIDTH + WIDTH * WIDTH + WIDTH + WIDTH * D_OUT + D_OUT
 PACKED_BYTES = 2 * (WIDTH * D_IN + WIDTH * WIDTH + D_OUT * WIDTH) + 4 * (2 * WIDTH + D_OUT)
 K_MAX = D_OUT
-# ALS iterations of the learned start; the plain start runs ALS to its own
-# tolerance (up to 1000 iterations, adamixture/entry.py:68-69).
-ALS_ITERS = 20
+# ALS iterations behind the learned start's features: None = ALS to its own
+# tolerance, as the plain start (up to 1000 iterations,
+# adamixture/entry.py:68-69); an integer truncates it (the network was also
+# trained in that mode, which pays off only while ALS dominates the start).
+ALS_ITERS = None
 # (name, rows, cols) of the flat parameter vector, in order.
 BLOCKS = [("W1", D_IN, WIDTH), ("b1", 1, WIDTH), ("W2", WIDTH, WIDTH), ("b2", 1, WIDTH),
           ("W3", WIDTH, D_OUT), ("b3", 1, D_OUT)]
@@ -292,29 +292,32 @@ def load_meta(path=WEIGHTS_PATH) -> dict:
 # ---------------------------------------------------------------------------
 
 def learned_warm_start(G: PackedGenotypes, U2, S2, V2, f, K: int, model: WarmStartMLP, seed: int, *,
-                       als_iters: int = ALS_ITERS, max_als: int = 1000, tol_als: float = 1e-4):
-    """Warm start from the rank-2K SVD (U2, S2, V2): ``als_iters`` ALS
-    iterations on its first K components (instead of ALS to its tolerance),
-    the network's Q, and P from Q by one decode-fused pass over G.
-    Returns (P, Q, logL, used_learned).  If the learned start's FP64 logL does
-    not exceed that of the short ALS start, the plain start is returned
-    instead: ALS continued from the short run to its own stopping rule."""
+                       als_iters: int | None = ALS_ITERS, max_als: int = 1000, tol_als: float = 1e-4):
+    """Warm start from the rank-2K SVD (U2, S2, V2): ALS on its first K
+    components (to tolerance, or ``als_iters`` iterations), the network's Q,
+    and P from Q by one decode-fused pass over G.  Returns
+    (P, Q, logL, used_learned).  If the learned start's FP64 logL does not
+    exceed that of the ALS start, the plain SVD + ALS start is returned
+    (ALS continued to its tolerance first when it was truncated)."""
     from .als_gpu import ALS_gpu
     from .em_adam_gpu import em_pass
     t0 = time.time()
     dev = G.device
     Uk, Sk, Vk = U2[:, :K].contiguous(), S2[:K].contiguous(), V2[:, :K].contiguous()
-    P_s, Q_s = ALS_gpu(Uk, Sk, Vk, f, seed, G.M, K, als_iters, tol_als, dev)
+    n_als = max_als if als_iters is None else als_iters
+    P_s, Q_s = ALS_gpu(Uk, Sk, Vk, f, seed, G.M, K, n_als, tol_als, dev)
     X = features(U2, S2, V2, f, P_s, Q_s, K)
     Q = model.infer(X, K)
     P = p_from_q_data(G, f, Q)
     ll = em_pass(G, P, Q, loglik=True, stats=False).item()
     ll_s = em_pass(G, P_s.contiguous(), Q_s.contiguous(), loglik=True, stats=False).item()
     used = ll > ll_s
-    log.info(f"    Learned warm start logL {ll:.1f} vs {als_iters}-iteration ALS {ll_s:.1f} -> "
+    log.info(f"    Learned warm start logL {ll:.1f} vs SVD+ALS {ll_s:.1f} -> "
              f"{'learned' if used else 'fallback to SVD+ALS'} ({time.time() - t0:.3f}s)")
     if used:
         return P, Q, ll, True
+    if als_iters is None:
+        return P_s, Q_s, ll_s, False
     P_a, Q_a = ALS_gpu(Uk, Sk, Vk, f, seed, G.M, K, max(max_als - als_iters, 0), tol_als, dev, P_init=P_s,
                        Q_init=Q_s)
     return P_a, Q_a, em_pass(G, P_a.contiguous(), Q_a.contiguous(), loglik=True, stats=False).item(), False

@@ -75,19 +75,22 @@ def align_columns(Q_ref: torch.Tensor, Q: torch.Tensor) -> list[int]:
 
 
 def make_training_set(seed: int, N: int, M: int, K: int, fst: float, alpha: float, miss: float, device,
-                      precision: str = "f64", als_iters: int = ALS_ITERS):
-    """(X, T, info) of one synthetic data set: features N x 64 from the
-    ``als_iters``-iteration ALS start, targets N x 16 = the Q the plain
-    method converges to (SVD + ALS to tolerance + Adam-EM), zero-padded, its
-    columns aligned to the short ALS start's clusters."""
+                      precision: str = "f64", als_iters: int | None = ALS_ITERS):
+    """(X, T, info) of one synthetic data set: features N x 64 from the ALS
+    start (to tolerance, or ``als_iters`` iterations), targets N x 16 = the
+    Q the plain method converges to (SVD + ALS to tolerance + Adam-EM),
+    zero-padded, its columns aligned to the feature start's clusters."""
     dev = torch.device(device)
     w = Workload(f"train{seed}", N, M, K, "bn", alpha, miss, fst=fst, seed=seed)
     t0 = time.time()
     G, _, _ = make_genotypes(w, dev)
     f = allele_frequencies(G)
     U, S, V, P_als, Q_als = plain_start(G, f, K, seed, dev)
-    P_s, Q_s = ALS_gpu(U[:, :K].contiguous(), S[:K].contiguous(), V[:, :K].contiguous(), f, seed, G.M, K,
-                       als_iters, SVD_DEFAULTS["tol_als"], dev)
+    if als_iters is None:
+        P_s, Q_s = P_als, Q_als
+    else:
+        P_s, Q_s = ALS_gpu(U[:, :K].contiguous(), S[:K].contiguous(), V[:, :K].contiguous(), f, seed, G.M, K,
+                           als_iters, SVD_DEFAULTS["tol_als"], dev)
     X = features(U, S, V, f, P_s, Q_s, K)
     n_checks = []
     P_star, Q_star = converge_plain(G, P_als, Q_als, K, dev, precision, on_check=lambda it, ll: n_checks.append(it))
@@ -268,9 +271,25 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
     ap.add_argument("--budget-s", type=float, default=0.0, help="stop generating data sets after this many seconds")
     ap.add_argument("--eval-sets", type=int, default=3, help="held-out data sets raced after training")
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--weights", default=None, help="skip training: race these weights on --eval-workload")
+    ap.add_argument("--eval-workload", default="learned_eval", help="synth.WORKLOADS entry raced with --weights")
+    ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     dev = _require_gpu("cuda:0")
+    if args.weights:
+        from .synth import WORKLOADS
+        model = WarmStartMLP.load(dev, args.weights)
+        w = WORKLOADS[args.eval_workload]
+        t0 = time.time()
+        G, _, _ = make_genotypes(w, dev)
+        f = allele_frequencies(G)
+        print(f"[learned] {w.name}: N={w.N} M={w.M} K={w.K} generated in {time.time() - t0:.1f}s", flush=True)
+        r = race(G, f, w.K, w.seed, dev, model, plain_precision="f32", fast_precision="bf16")
+        r["workload"] = w.name
+        print(json.dumps(r), flush=True)
+        return 0
+    if not args.out:
+        ap.error("--out is required when training")
     rng = np.random.default_rng(args.seed)
     Xs, Ts, Ks, infos = [], [], [], []
     t_start = time.time()

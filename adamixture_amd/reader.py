@@ -103,6 +103,53 @@ def read_bed_packed(path: str | Path, chrom_mode: str = "all", autosomes: int = 
     return torch.from_numpy(Gp), N, M
 
 
+def read_bed_gpu(path: str | Path, device="cuda", chrom_mode: str = "all", autosomes: int = 22,
+                 specific_chrom=None, block_bytes: int = 1 << 29):
+    """read_bed_packed with the transpose, mean and flip on the GPU
+    (adx_bed_to_packed / adx_packed_allele_sum / adx_flip_packed): the .bed
+    is memory-mapped, the kept SNP rows are copied to the device in blocks of
+    about ``block_bytes`` and transposed straight into the padded device
+    layout.  Returns (PackedGenotypes, N, M)."""
+    from . import _lib
+    from .genotypes import PackedGenotypes, padded_ld
+    path = Path(path)
+    base = path.with_suffix("") if path.suffix == ".bed" else path
+    dev = torch.device(device)
+    with open(f"{base}.fam") as f:
+        N = sum(1 for _ in f)
+    with open(f"{base}.bim") as f:
+        keep = np.array([_keep_chrom(line.split()[0], chrom_mode, autosomes, specific_chrom)
+                         for line in f if line.strip()], dtype=bool)
+    nb = ceil(N / 4)
+    raw = np.memmap(f"{base}.bed", dtype=np.uint8, mode="r")
+    if raw.size < 3 or raw[0] != 0x6C or raw[1] != 0x1B or raw[2] != 0x01:
+        raise ValueError(f"{base}.bed is not a SNP-major PLINK bed file")
+    if (raw.size - 3) % nb or (raw.size - 3) // nb != keep.size:
+        raise ValueError("bim file doesn't match!")
+    bed = raw[3:].reshape(-1, nb)
+    idx = np.flatnonzero(keep)
+    M = idx.size
+    if M == 0 or N == 0:
+        raise ValueError("genotype data contains 0 SNPs or 0 samples after filtering")
+    ldg = padded_ld(N)
+    data = torch.empty(((M + 3) // 4, ldg), dtype=torch.uint8, device=dev)
+    block_snps = max(4, (block_bytes // nb) // 4 * 4)
+    st = _lib.stream_handle(dev)
+    contiguous = M == keep.size
+    for s in range(0, M, block_snps):
+        e = min(M, s + block_snps)
+        rows = np.asarray(bed[s:e]) if contiguous else bed[idx[s:e]]
+        dbed = torch.from_numpy(np.ascontiguousarray(rows)).to(dev)
+        _lib.call("adx_bed_to_packed", dbed.data_ptr(), nb, e - s, N, data.data_ptr(), ldg, s // 4, st)
+    sums = torch.empty(2, dtype=torch.int64, device=dev)     # uint64 counters, < 2^63
+    _lib.call("adx_packed_allele_sum", data.data_ptr(), ldg, M, N, sums.data_ptr(), st)
+    s_all, c_all = (int(x) for x in sums.cpu().tolist())
+    if c_all and s_all / c_all >= 0.5:
+        log.info("    Flipping genotype encoding (packed).")
+        _lib.call("adx_flip_packed", data.data_ptr(), ldg, M, N, st)
+    return PackedGenotypes(data, M, N), N, M
+
+
 def write_bed(base: str | Path, G: np.ndarray) -> None:
     """Write an unpacked M x N genotype matrix (codes 0/1/2, 3 missing) as
     PLINK .bed/.bim/.fam (code map inverse of the reader's)."""

@@ -59,6 +59,23 @@ int adx_freq_f32(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, float *f, 
  * 2 x #observed genotypes of sample i (FP64, exact). */
 int adx_qbat(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, double *qbat, void *stream);
 
+/* GPU form of the reader's transpose (adamixture/src/utils_c/cython/
+ * snp_reader.pyx:76-120): `rows` consecutive SNP-major BED rows of nb bytes
+ * (4 samples per byte, codes 00 -> 2, 01 -> 3 missing, 10 -> 1, 11 -> 0)
+ * become output byte rows row0 .. row0 + ceil(rows/4) - 1 of G (4 SNPs per
+ * byte); fields beyond `rows` are 0 and bytes i >= N are 0xFF. */
+int adx_bed_to_packed(const uint8_t *bed, int64_t nb, int64_t rows, int64_t N, uint8_t *G, int64_t ldg,
+                      int64_t row0, void *stream);
+
+/* get_mean_packed (snp_reader.pyx:215-251): out2[0] = sum of observed allele
+ * counts, out2[1] = 2 x observed genotypes (exact, device uint64). */
+int adx_packed_allele_sum(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, unsigned long long *out2,
+                          void *stream);
+
+/* flip_packed (snp_reader.pyx:254-302): 0 <-> 2 in place, missing kept,
+ * fields of the last byte row beyond M cleared. */
+int adx_flip_packed(uint8_t *G, int64_t ldg, int64_t M, int64_t N, void *stream);
+
 /* No reference counterpart (layout): copy Mb rows of N bytes from a tensor
  * with row stride ld_src into rows of ldg bytes, padding bytes set to 0xFF. */
 int adx_repack_padded(const uint8_t *src, int64_t ld_src, int64_t Mb, int64_t N, uint8_t *dst,
