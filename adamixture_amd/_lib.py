@@ -57,6 +57,13 @@ SIGNATURES = {
     "adx_bed_to_packed": (_int, [_p, _i64, _i64, _i64, _p, _i64, _i64, _p]),
     "adx_packed_allele_sum": (_int, [_p, _i64, _i64, _i64, _p, _p]),
     "adx_flip_packed": (_int, [_p, _i64, _i64, _i64, _p]),
+    "adx_label_freq": (_int, [_p, _i64, _i64, _i64, _int, _p, _p, _dbl, _p, _p]),
+    "adx_snap_labels": (_int, [_p, _i64, _int, _p, _dbl, _p]),
+    "adx_cv_mask_list": (_int, [_p, _i64, _i64, _i64, _p, _int, _i64, _i64, _i64, _p]),
+    "adx_cv_mask_hash": (_int, [_p, _p, _i64, _i64, _i64, _u64, _int, _int, _p]),
+    "adx_cv_deviance_list": (_int, [_p, _i64, _i64, _i64, _p, _int, _i64, _i64, _i64, _p, _p, _int, _p, _p, _p]),
+    "adx_cv_deviance_hash": (_int, [_p, _i64, _i64, _i64, _u64, _int, _int, _p, _p, _int, _p, _p, _p]),
+    "adx_project_simplex": (_int, [_p, _i64, _int, _dbl, _p]),
     "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p]),
     "adx_gemm_nn_f64": (_int, [_p, _i64, _i64, _int, _int, _p, _i64, _dbl, _p, _p, _dbl, _p, _i64, _p]),
     "adx_set_em_tiles": (_int, [_int, _int, _i64]),

@@ -62,8 +62,12 @@ def _loglik(G: PackedGenotypes, P, Q) -> float:
 
 
 def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patience, device, chunk_size,
-                                threads_per_block, on_iter=None):
-    """br_qn_gpu.py:217-382."""
+                                threads_per_block, on_iter=None, snap=None):
+    """br_qn_gpu.py:217-382.  ``snap(Q)``, if given, is applied to every Q
+    the iteration produces -- after each SQP Q solve and after the
+    quasi-Newton map -- as the supervised mode does (supervised.py:689, 697,
+    741)."""
+    snap = snap or (lambda X: None)
     dev = torch.device(device)
     G = as_packed(G, M, dev)
     vkk = v_kk(K, dev)
@@ -79,6 +83,7 @@ def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patienc
     def sweep(Qc, Pc):
         Hq, zq = grad_hess(G, Qc, Pc, "q")
         Qn = sqp_solve_q(Hq, zq.neg_(), Qc, vkk, N, K)
+        snap(Qn)
         Hp, zp = grad_hess(G, Qn, Pc, "p")
         Pn = sqp_solve_p(Hp, zp.neg_(), Pc, M, K)
         return Pn, Qn
@@ -105,6 +110,7 @@ def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patienc
         P_qn = x_qn[:M * K].view(M, K).clone()
         Q_qn = x_qn[M * K:].view(N, K).clone()
         _mapPQ_gpu(P_qn, Q_qn)
+        snap(Q_qn)
         ll_qn = _loglik(G, P_qn, Q_qn)
         if ll_qn > ll_prev:
             P.copy_(P_qn)
@@ -136,7 +142,7 @@ def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patienc
 
 
 def optimize_original_gpu(G, P, Q, max_iter, K, M, N, tol, Q_hist, patience, device, chunk_size,
-                          threads_per_block, on_iter=None):
+                          threads_per_block, on_iter=None, snap=None):
     """br_qn_gpu.py:385-411 with the reference's OOM retry."""
     from .utils import MIN_GPU_CHUNK_SIZE, is_gpu_oom, reduce_gpu_chunk_or_raise
     cur = chunk_size
@@ -147,7 +153,7 @@ def optimize_original_gpu(G, P, Q, max_iter, K, M, N, tol, Q_hist, patience, dev
         Q_attempt.copy_(Q)
         try:
             return _optimize_original_gpu_once(G, P_attempt, Q_attempt, max_iter, K, M, N, tol, Q_hist, patience,
-                                               device, cur, threads_per_block, on_iter)
+                                               device, cur, threads_per_block, on_iter, snap)
         except RuntimeError as exc:
             if not is_gpu_oom(exc) or cur <= MIN_GPU_CHUNK_SIZE:
                 raise

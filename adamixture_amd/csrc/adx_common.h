@@ -70,6 +70,44 @@ struct Mma16<double> {
     __host__ __device__ static constexpr int row(int g, int j) { return g + 4 * j; }
 };
 
+// sqp.pyx:112-146 / sqp_kernel.cu:125-166 project_q_simplex_row, serial:
+// out = the Euclidean projection of x onto {v >= pc, sum v = 1}, by sorting
+// x - pc in descending order (insertion sort, K <= 64) and finding the
+// threshold.  b (K doubles) is scratch; out may alias x but not b.
+__device__ inline void project_simplex_row(const double* x, double* out, double* b, int K, double pc) {
+    int idx[64];
+    for (int i = 0; i < K; ++i) {
+        b[i] = x[i] - pc;
+        idx[i] = i;
+    }
+    for (int i = 1; i < K; ++i) {
+        const int key = idx[i];
+        const double kv = b[key];
+        int j = i - 1;
+        while (j >= 0 && b[idx[j]] < kv) {
+            idx[j + 1] = idx[j];
+            --j;
+        }
+        idx[j + 1] = key;
+    }
+    const double tau = 1.0 - K * pc;
+    double tsum = 0.0, tmax = 0.0;
+    bool bget = false;
+    for (int i = 0; i < K - 1; ++i) {
+        tsum += b[idx[i]];
+        tmax = (tsum - tau) / (i + 1);
+        if (tmax >= b[idx[i + 1]]) {
+            bget = true;
+            break;
+        }
+    }
+    if (!bget) tmax = (tsum + b[idx[K - 1]] - tau) / K;
+    for (int i = 0; i < K; ++i) {
+        const double v = b[i] - tmax;
+        out[i] = (v < 0.0 ? 0.0 : v) + pc;
+    }
+}
+
 // Inverse of row(): the lane group g and element j holding row r.
 template <typename T>
 __host__ __device__ constexpr int row_group(int r) {

@@ -90,7 +90,8 @@ template <typename T, int KS, int CW, bool STATS, bool LL>
 __global__ __launch_bounds__(256) void em_pass_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
     int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
-    T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart) {
+    T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart,
+    T* __restrict__ ABpart) {
     using MM = Mma16<T>;
     using acc_t = typename MM::acc;
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
@@ -106,11 +107,11 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
 
     __shared__ T s_tile[4][256];
-    __shared__ T s_tsum[64 * 16];
+    // Per wave: T of the 64 samples of the current row block over the wave's
+    // SNPs.  Each wave writes each (sample, k) once per row block, so the
+    // slices need no atomics and are summed in a fixed order at the flush.
+    __shared__ T s_tsum[STATS ? 4 : 1][64 * 16];
     __shared__ double s_ll[4];
-
-    if (STATS)
-        for (int e = threadIdx.x; e < 64 * 16; e += 256) s_tsum[e] = T(0);
 
     // P operands, constant for the whole strip.
     T pH[CW][KS];
@@ -224,27 +225,28 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                     v += __shfl_xor(v, 2);
                     v += __shfl_xor(v, 4);
                     v += __shfl_xor(v, 8);
-                    if (cl < K) atomicAdd(&s_tsum[(16 * g + 4 * rw + j) * 16 + cl], accT[j] + v);
+                    if (cl < K) s_tsum[wave][(16 * g + 4 * rw + j) * 16 + cl] = accT[j] + v;
                 }
             }
         }
 
         if (STATS) {
-            // T of these 64 samples over this workgroup's SNPs: a plain store
-            // into the workgroup's slice of Tpart (each (slice, sample) is
-            // written once; tpart_reduce_kernel sums the slices), or, without
-            // a workspace, a device-scope atomic (performed at the memory side).
+            // T of these 64 samples over this workgroup's SNPs (the four wave
+            // slices summed in a fixed order): a plain store into the
+            // workgroup's slice of Tpart (each (slice, sample) is written
+            // once; tpart_reduce_kernel sums the slices), or, without a
+            // workspace, a device-scope atomic (performed at the memory side).
             __syncthreads();
             for (int e = threadIdx.x; e < 64 * 16; e += 256) {
                 const int kp = e & 15;
                 const int64_t smp = s0 + (e >> 4);
                 if (kp < K && smp < N) {
+                    const T v = (s_tsum[0][e] + s_tsum[1][e]) + (s_tsum[2][e] + s_tsum[3][e]);
                     if (Tpart)
-                        Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = s_tsum[e];
+                        Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = v;
                     else
-                        atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+                        atomicAdd(&Tacc[smp * K + kp], v);
                 }
-                s_tsum[e] = T(0);
             }
             __syncthreads();
         }
@@ -260,8 +262,17 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                 for (int jj = 0; jj < 4; ++jj) {
                     const int kp = MM::row(g, jj);
                     if (kp < K) {
-                        atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
-                        atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
+                        // With a workspace: this sample split's slice of the A / B
+                        // partial sums (each element written by one workgroup),
+                        // summed in a fixed order by tpart_reduce_kernel.
+                        if (ABpart) {
+                            const int64_t mk = M * (int64_t)K;
+                            ABpart[(int64_t)blockIdx.y * mk + snp * K + kp] = accA[cb][jj];
+                            ABpart[((int64_t)gridDim.y + blockIdx.y) * mk + snp * K + kp] = accB[cb][jj];
+                        } else {
+                            atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
+                            atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
+                        }
                     }
                 }
             }
@@ -445,32 +456,38 @@ static EmGrid em_grid(int dtype, int64_t M, int64_t N) {
 template <typename T, int KS, bool STATS, bool LL>
 static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
                       int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B, T* Tm,
-                      double* llp, T* Tpart) {
+                      double* llp, T* Tpart, T* ABpart) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     if constexpr (sizeof(T) == 4) {  // the FP64 form has a single tile width
         if (gd.cw == 8) {
             em_pass_kernel<T, KS, 8, STATS, LL>
-                <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart);
+                <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart,
+                                       ABpart);
             return;
         }
     }
     em_pass_kernel<T, KS, 4, STATS, LL>
-        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart);
+        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart, ABpart);
 }
 
 template <typename T, bool STATS, bool LL>
 static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M,
                         int64_t N, int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B,
                         T* Tm, double* llp, T* Tpart) {
+    // Workspace after the T slices: A slices then B slices, one per sample split.
+    T* ABpart = Tpart ? Tpart + gd.nbx * N * (int64_t)K : nullptr;
     switch ((K + 3) / 4) {
-        case 1: launch_em<T, 1, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
-        case 2: launch_em<T, 2, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
-        case 3: launch_em<T, 3, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
-        default: launch_em<T, 4, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart); break;
+        case 1: launch_em<T, 1, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
+        case 2: launch_em<T, 2, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
+        case 3: launch_em<T, 3, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
+        default: launch_em<T, 4, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
     }
     if (STATS && Tpart) {
-        const int64_t nk = N * K;
+        const int64_t nk = N * K, mk = M * K;
         tpart_reduce_kernel<T><<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, Tm);
+        tpart_reduce_kernel<T><<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart, gd.splits, mk, A);
+        tpart_reduce_kernel<T><<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart + gd.splits * mk, gd.splits, mk,
+                                                                           B);
     }
 }
 
@@ -504,7 +521,8 @@ int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
     if (M <= 0 || N <= 0) return 0;
     const EmGrid gd = em_grid(dtype, M, N);
     const int64_t esz = dtype == ADX_F64 ? 8 : 4;
-    return ll_part_bytes(gd) + esz * gd.nbx * N * (K > 0 ? K : 0);
+    const int64_t k = K > 0 ? K : 0;
+    return ll_part_bytes(gd) + esz * (gd.nbx * N * k + 2 * gd.splits * M * k);
 }
 
 int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,

@@ -70,7 +70,8 @@ template <int KH, int CW>
 __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
     const __bf16* __restrict__ Qx, const __bf16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
-    float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc, float* __restrict__ Tpart) {
+    float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc, float* __restrict__ Tpart,
+    float* __restrict__ ABpart) {
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
     constexpr int LX = 32 * KH;  // row length of Px / Qx
     constexpr int TS = 20;       // row stride (words) of the per-wave transpose tile
@@ -86,9 +87,10 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
 
     // Per wave: word [sample row r][SNP column c] = bf16 a | bf16 b << 16.
     __shared__ __attribute__((aligned(16))) uint32_t s_tile[4][16 * TS];
-    __shared__ float s_tsum[64 * 16];
-
-    for (int e = threadIdx.x; e < 64 * 16; e += 256) s_tsum[e] = 0.0f;
+    // Per wave: T of the 64 samples of the current row block over the wave's
+    // SNPs; each (sample, k) is written once per wave and row block, and the
+    // four slices are summed in a fixed order at the flush (deterministic).
+    __shared__ float s_tsum[4][64 * 16];
 
     const bf16x8 zero8 = bf16x8{};
     bf16x8 pH[CW][KH];
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (cl < K) atomicAdd(&s_tsum[(16 * g + 4 * (2 * tp + rr) + j) * 16 + cl], accT[rr][j]);
+                    if (cl < K) s_tsum[wave][(16 * g + 4 * (2 * tp + rr) + j) * 16 + cl] = accT[rr][j];
         }
 
         __syncthreads();
@@ -229,12 +231,12 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             const int kp = e & 15;
             const int64_t smp = s0 + (e >> 4);
             if (kp < K && smp < N) {
+                const float v = (s_tsum[0][e] + s_tsum[1][e]) + (s_tsum[2][e] + s_tsum[3][e]);
                 if (Tpart)
-                    Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = s_tsum[e];
+                    Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = v;
                 else
-                    atomicAdd(&Tacc[smp * K + kp], s_tsum[e]);
+                    atomicAdd(&Tacc[smp * K + kp], v);
             }
-            s_tsum[e] = 0.0f;
         }
         __syncthreads();
     }
@@ -248,8 +250,16 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             for (int jj = 0; jj < 4; ++jj) {
                 const int kp = 4 * g + jj;
                 if (kp < K) {
-                    atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
-                    atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
+                    // this sample split's slice of the A / B partial sums (each
+                    // element written by one workgroup), summed in a fixed order
+                    if (ABpart) {
+                        const int64_t mk = M * (int64_t)K;
+                        ABpart[(int64_t)blockIdx.y * mk + snp * K + kp] = accA[cb][jj];
+                        ABpart[((int64_t)gridDim.y + blockIdx.y) * mk + snp * K + kp] = accB[cb][jj];
+                    } else {
+                        atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
+                        atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
+                    }
                 }
             }
         }
@@ -328,7 +338,8 @@ static Bf16Grid bf16_grid(int64_t M, int64_t N) {
     return gd;
 }
 
-// T[e] += sum over the nparts slices of Tpart[p * nk + e] (nk = N K).
+// T[e] += sum over the nparts slices of Tpart[p * nk + e] (also used for the
+// A and B slices, nk = M K).
 __global__ void tpart_reduce_kernel(const float* __restrict__ Tpart, int64_t nparts, int64_t nk,
                                     float* __restrict__ T) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -350,14 +361,17 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
                         int K, const __bf16* Px, const __bf16* Qx, const __bf16* Qt, int64_t ldqt, float* A,
                         float* B, float* T, float* Tpart) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
+    float* ABpart = Tpart + gd.nbx * N * (int64_t)K;
     if (gd.cw == 8)
         em_pass_bf16_kernel<KH, 8>
-            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart);
+            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart, ABpart);
     else
         em_pass_bf16_kernel<KH, 4>
-            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart);
-    const int64_t nk = N * K;
+            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart, ABpart);
+    const int64_t nk = N * K, mk = M * K;
     tpart_reduce_kernel<<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, T);
+    tpart_reduce_kernel<<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart, gd.splits, mk, A);
+    tpart_reduce_kernel<<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart + gd.splits * mk, gd.splits, mk, B);
 }
 
 // Byte offset of the partial-T slices in the operand buffer, and its size.
@@ -378,7 +392,7 @@ int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t* ldqt) {
     if (ldqt) *ldqt = ld;
     if (M <= 0 || N <= 0) return bf16_base_bytes(M > 0 ? M : 0, N > 0 ? N : 0, K);
     const Bf16Grid gd = bf16_grid(M, N);
-    return bf16_base_bytes(M, N, K) + (int64_t)sizeof(float) * gd.nbx * N * K;
+    return bf16_base_bytes(M, N, K) + (int64_t)sizeof(float) * (gd.nbx * N * K + 2 * gd.splits * M * K);
 }
 
 int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int K, void* operands,

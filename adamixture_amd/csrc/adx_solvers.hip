@@ -206,39 +206,8 @@ __global__ __launch_bounds__(64) void sqp_kernel(const double* __restrict__ H, c
         if (SIMPLEX) {
             // sqp_kernel.cu:125-166 project_q_simplex_row (serial, K <= 64).
             if (lane == 0) {
-                const double pc = 1e-5;
-                double* b = t1;
-                int idx[64];
-                for (int i = 0; i < K; ++i) {
-                    b[i] = par[i] + delta[i] - pc;
-                    idx[i] = i;
-                }
-                for (int i = 1; i < K; ++i) {
-                    const int key = idx[i];
-                    const double kv = b[key];
-                    int j = i - 1;
-                    while (j >= 0 && b[idx[j]] < kv) {
-                        idx[j + 1] = idx[j];
-                        --j;
-                    }
-                    idx[j + 1] = key;
-                }
-                const double tau = 1.0 - K * pc;
-                double tsum = 0.0, tmax = 0.0;
-                bool bget = false;
-                for (int i = 0; i < K - 1; ++i) {
-                    tsum += b[idx[i]];
-                    tmax = (tsum - tau) / (i + 1);
-                    if (tmax >= b[idx[i + 1]]) {
-                        bget = true;
-                        break;
-                    }
-                }
-                if (!bget) tmax = (tsum + b[idx[K - 1]] - tau) / K;
-                for (int i = 0; i < K; ++i) {
-                    const double v = b[i] - tmax;
-                    out[i] = (v < 0.0 ? 0.0 : v) + pc;
-                }
+                for (int i = 0; i < K; ++i) out[i] = par[i] + delta[i];
+                project_simplex_row(out, out, t1, K, 1e-5);
             }
         } else {
             for (int i = lane; i < K; i += 64) {

@@ -137,6 +137,13 @@ class EMAdamOptimizer:
                   _lib.stream_handle(self.device))
         return self.P_EM, self.Q_EM
 
+    def finalize_q(self, G: PackedGenotypes, Q: torch.Tensor) -> torch.Tensor:
+        """Q half of the EM finalisation only (projection.py:271-272 discards P_EM)."""
+        _lib.call("adx_em_finalize", self.acc_dtype, None, Q.data_ptr(), None, None, self.T_accum.data_ptr(),
+                  G.qbat().data_ptr(), 0, Q.shape[0], Q.shape[1], None, self.Q_EM.data_ptr(),
+                  _lib.stream_handle(self.device))
+        return self.Q_EM
+
     def run_em_step(self, G, P: torch.Tensor, Q: torch.Tensor, M: int, chunk_size: int = 0,
                     unpacker=None) -> tuple[torch.Tensor, torch.Tensor]:
         """em_adam_gpu.py:166-199: one full EM sweep -> (P_EM, Q_EM)."""
@@ -156,6 +163,15 @@ class EMAdamOptimizer:
                   self.lr, self.beta1, self.beta2, self.reg_adam, None, None, 0,
                   _lib.stream_handle(self.device))
 
+    def step_q(self, Q: torch.Tensor, Q_target: torch.Tensor) -> None:
+        """projection.py:275-290: Adam on Q only, clamp, renormalise."""
+        if Q.dtype != torch.float64 or not Q.is_contiguous():
+            raise TypeError("Q must be a contiguous float64 tensor")
+        self.t += 1.0
+        _lib.call("adx_adam_step", None, Q.data_ptr(), None, Q_target.contiguous().data_ptr(), None, None,
+                  self.m_Q.data_ptr(), self.v_Q.data_ptr(), 0, Q.shape[0], Q.shape[1], self.t, self.lr,
+                  self.beta1, self.beta2, self.reg_adam, None, None, 0, _lib.stream_handle(self.device))
+
 
 def loglikelihood_gpu(G, P: torch.Tensor, Q: torch.Tensor, M: int, N: int, batch_size: int = 0,
                       device: torch.device | None = None, threads_per_block: int = 0) -> float:
@@ -169,13 +185,14 @@ def loglikelihood_gpu(G, P: torch.Tensor, Q: torch.Tensor, M: int, N: int, batch
 
 def _optimize_parameters_gpu_once(G, P, Q, lr, beta1, beta2, reg_adam, max_iter, check, K, M, N, lr_decay,
                                   min_lr, patience, tol_adam, device, chunk_size, threads_per_block,
-                                  precision=None, on_check=None):
+                                  precision=None, on_check=None, after_step=None):
     """em_adam_gpu.py:281-356 with the log-likelihood of each check fused into
     the next EM pass (which runs at the same (P, Q)): the decision taken at
     check iteration ``it`` happens before the Adam step of iteration it+1,
     so the trajectory, the lr schedule and the returned best (P, Q) are the
     reference's; at most one EM pass is computed and discarded at the end.
-    ``on_check(it, logl)`` is called at every evaluation."""
+    ``on_check(it, logl)`` is called at every evaluation; ``after_step(P, Q)``
+    after every Adam step (the supervised mode snaps labelled Q rows there)."""
     del K, N, chunk_size, threads_per_block
     device = torch.device(device)
     G = as_packed(G, M, device)
@@ -184,6 +201,8 @@ def _optimize_parameters_gpu_once(G, P, Q, lr, beta1, beta2, reg_adam, max_iter,
     opt.accumulate(G, P, Q)
     opt.finalize(G, P, Q)
     opt.step(P, Q, opt.P_EM, opt.Q_EM)
+    if after_step is not None:
+        after_step(P, Q)
     log.info(f"    Performed priming iteration... ({time.time() - ts:.1f}s)\n")
 
     P_best = P.clone()
@@ -230,6 +249,8 @@ def _optimize_parameters_gpu_once(G, P, Q, lr, beta1, beta2, reg_adam, max_iter,
             pending = None
         opt.finalize(G, P, Q)
         opt.step(P, Q, opt.P_EM, opt.Q_EM)
+        if after_step is not None:
+            after_step(P, Q)
         if (it + 1) % check == 0:
             pending = it + 1
     if not stopped and pending is not None:
@@ -241,7 +262,7 @@ def _optimize_parameters_gpu_once(G, P, Q, lr, beta1, beta2, reg_adam, max_iter,
 
 def optimize_parameters_gpu(G, P, Q, lr, beta1, beta2, reg_adam, max_iter, check, K, M, N, lr_decay, min_lr,
                             patience, tol_adam, device, chunk_size, threads_per_block, precision=None,
-                            on_check=None):
+                            on_check=None, after_step=None):
     """em_adam_gpu.py:359-385: Adam-EM with the reference's OOM retry (the
     retry restarts from the unmodified P, Q; the fused pass has no chunk
     temporaries, so the halved chunk size changes nothing but is kept)."""
@@ -255,7 +276,7 @@ def optimize_parameters_gpu(G, P, Q, lr, beta1, beta2, reg_adam, max_iter, check
         try:
             return _optimize_parameters_gpu_once(G, P_attempt, Q_attempt, lr, beta1, beta2, reg_adam, max_iter,
                                                  check, K, M, N, lr_decay, min_lr, patience, tol_adam, device,
-                                                 cur, threads_per_block, precision, on_check)
+                                                 cur, threads_per_block, precision, on_check, after_step)
         except RuntimeError as exc:
             if not is_gpu_oom(exc) or cur <= MIN_GPU_CHUNK_SIZE:
                 raise

@@ -7,7 +7,13 @@ the fused decode+EM-statistics pass, EM finalize, Adam step, and every 5th
 step the FP64 log-likelihood check (fused into the next pass in the f64
 form, a separate FP64 pass in the f32 form) with its host read-back.
 
-Prints one JSON line (see DESIGN.md for the roofline and baseline figures).
+Then, unless --no-ttt, the workload's headline race: seconds from the start
+of initialisation (randomized SVD + ALS, or the learned warm start) to the
+first FP64 logL check within 1e-6 relative of the converged plain FP32
+result, for the plain FP32, plain BF16 and learned-start BF16 forms
+("time_to_target" in the JSON line).
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -42,8 +48,8 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default EM tile shape instead of timing the candidates first")
     ap.add_argument("--tiles", default=None, help="fixed EM tile shape 'cw/target_wgs' (implies --no-autotune)")
-    ap.add_argument("--ttt", action="store_true",
-                    help="also race time to logL within 1e-6 of the plain FP32 result (adds ~2 min)")
+    ap.add_argument("--no-ttt", action="store_true",
+                    help="skip the time-to-target race (plain FP32 vs BF16 vs learned start + BF16, ~1 min)")
     return ap.parse_args()
 
 
@@ -74,7 +80,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the EM pass would shard by "
-              "SNPs with one all-reduce of T and q_bat per iteration, see DESIGN.md)", file=sys.stderr)
+              "SNPs with one all-reduce of T and q_bat per iteration)", file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -198,7 +204,7 @@ def main():
     if not args.no_cpu_baseline:
         rows = args.cpu_rows or max(4, min(M, int(4e9 / N)))
         line["cpu_baseline"] = cpu_baseline(G, P0, Q0, w, rows)
-    if args.ttt:
+    if not args.no_ttt:
         del opt, P, Q
         line["time_to_target"] = time_to_target(G, w, dev)
     print(json.dumps(line), flush=True)

@@ -100,8 +100,9 @@ int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs);
 int adx_get_em_tiles(int dtype, int *cw, int64_t *target_wgs);
 
 /* Workspace (bytes) of adx_em_pass for the given shape, dtype and current
- * tile shape: the log-likelihood partial sums plus the per-workgroup partial
- * sums of T (ceil(M / (64 cw)) x N x K of the pass dtype). */
+ * tile shape: the log-likelihood partial sums, the per-workgroup partial sums
+ * of T (ceil(M / (64 cw)) x N x K of the pass dtype) and the per-sample-split
+ * partial sums of A and B (2 x splits x M x K). */
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
 
 /* Replaces the chunk loop of EMAdamOptimizer.run_em_step with
@@ -112,9 +113,10 @@ int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
  *                  (N x K), with H = P Q^T clamped to [1e-5, 1-1e-5],
  *                  a = g/H, b = (2-g)/(1-H) on observed genotypes.  A, B, T
  *                  are accumulated (caller zeroes them; adx_em_finalize does).
- *                  With a workspace of adx_em_workspace_bytes, T is reduced
- *                  through per-workgroup partial sums (plain stores and one
- *                  summing kernel); with less, by device-scope atomics.
+ *                  With a workspace of adx_em_workspace_bytes, T, A and B
+ *                  are reduced through partial sums (plain stores, summed in
+ *                  a fixed order: deterministic); with less, by device-scope
+ *                  atomics.
  *   ADX_EM_LOGLIK: *ll_out (device, FP64) = sum g log H + (2-g) log1p(-H),
  *                  H clamped to [1e-10, 1-1e-10]; FP64 dtype only.
  * dtype selects the arithmetic of P, Q, A, B, T: ADX_F64 or ADX_F32. */
@@ -128,9 +130,11 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
  * that adx_em_prep_bf16 builds from the FP64 P and Q (H from a three-term
  * hi/lo split; A and B from Q split in hi + lo; a, b rounded to BF16).
  * The operand buffer also holds the per-workgroup partial sums of T
- * (ceil(M / (64 cw)) x N x K FP32, cw from adx_set_em_tiles), so its size
+ * (ceil(M / (64 cw)) x N x K FP32, cw from adx_set_em_tiles) and the
+ * per-sample-split partial sums of A and B (2 x splits x M x K), so its size
  * depends on the current tile shape: query adx_em_bf16_operand_bytes after
- * changing it.  Replaces the same reference code (em_adam_gpu.py:46-78,
+ * changing it.  The slices are summed in a fixed order: the pass is
+ * deterministic.  Replaces the same reference code (em_adam_gpu.py:46-78,
  * :166-197). */
 int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t *ldqt);
 int adx_em_prep_bf16(const double *P, const double *Q, int64_t M, int64_t N, int K, void *operands,
@@ -290,6 +294,63 @@ int adx_mlp_pack_bf16(const float *params, void *packed, void *stream);
  * reference maps Q (adamixture/model/als_gpu.py:23-36).  2 <= K <= 16. */
 int adx_mlp_infer_bf16(const float *X, int64_t ldx, int64_t N, int K, const void *packed, double *Q, int64_t ldq,
                        void *stream);
+
+/* --- supervised mode ------------------------------------------------------ */
+
+/* Replaces init_p_supervised_packed (adamixture/src/supervised.py:84-107):
+ * P (M x K, FP64, row stride K) = per-population allele frequencies of the
+ * labelled samples, 0.5 where a population has no observed genotype at a
+ * SNP, clipped to [eps, 1 - eps]; bit-exact with the reference.  order holds
+ * the labelled sample indices (each in [0, N)) grouped by population,
+ * population k (0-based) at order[offsets[k] .. offsets[k + 1]); ascending
+ * indices inside a group read best.  order and offsets are device arrays.
+ * N < 2^30. */
+int adx_label_freq(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, const int32_t *order,
+                   const int64_t *offsets, double eps, double *P, void *stream);
+
+/* Replaces _snap_q_gpu (supervised.py:308-339): for every sample with
+ * y[i] > 0, Q[i, :] = eps / max(K - 1, 1) and Q[i, y[i] - 1] = 1 - eps;
+ * rows with y[i] <= 0 are unchanged.  y is a device int32 array. */
+int adx_snap_labels(double *Q, int64_t N, int K, const int32_t *y, double eps, void *stream);
+
+/* --- cross-validation ------------------------------------------------------ */
+
+/* Partial-sum slots the deviance entry points need in `work` (doubles). */
+#define ADX_CV_BLOCKS 1024
+
+/* A fold given as a list of flat indices f = m N + i (int32 if idx64 == 0,
+ * else int64), entries flat[first + e * stride] for e < n (the reference's
+ * shuffled array and fold f = shuffled[f::n_folds], adamixture/src/cv.py:
+ * 214-231).  Every f must lie in [0, M N).  G is a device copy of the packed
+ * matrix (row stride ldg, a multiple of 4); the held-out 2-bit fields are
+ * set to 3 (missing) in place -- replaces mask_entries_i32 / _i64
+ * (br_qn.pyx:150-203) on the packed layout. */
+int adx_cv_mask_list(uint8_t *G, int64_t ldg, int64_t M, int64_t N, const void *flat, int idx64, int64_t first,
+                     int64_t stride, int64_t n, void *stream);
+
+/* A fold given by hashing (cv.py:61-81, the streaming path): entry (m, i)
+ * is held out iff its genotype is observed and splitmix64(m N + i + seed)
+ * mod n_folds == fold.  Gm (same shape as G) receives G with those fields
+ * set to 3. */
+int adx_cv_mask_hash(const uint8_t *G, uint8_t *Gm, int64_t ldg, int64_t M, int64_t N, uint64_t seed, int n_folds,
+                     int fold, void *stream);
+
+/* Replace deviance_squared_sum / _i32 (br_qn.pyx:92-149, 205-246): *out
+ * (device) = sum over the fold's observed entries of G of g log(g / mu) +
+ * (2 - g) log((2 - g) / (2 - mu)), mu = 2 Q_i . P_m clamped to
+ * [1e-10, 2 - 1e-10].  P (M x K) and Q (N x K) FP64 row-major; work holds
+ * ADX_CV_BLOCKS doubles.  Fixed summation order (deterministic). */
+int adx_cv_deviance_list(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const void *flat, int idx64,
+                         int64_t first, int64_t stride, int64_t n, const double *P, const double *Q, int K,
+                         double *work, double *out, void *stream);
+int adx_cv_deviance_hash(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, uint64_t seed, int n_folds, int fold,
+                         const double *P, const double *Q, int K, double *work, double *out, void *stream);
+
+/* project_q_simplex (sqp.pyx:112-146) on n rows of X (n x K, FP64) in place:
+ * each row becomes its Euclidean projection onto {x >= pc, sum x = 1}.
+ * The quasi-Newton map of the CPU optimizer (br_qn.py:159-160), which the
+ * cross-validation polish uses.  K <= 64. */
+int adx_project_simplex(double *X, int64_t n, int K, double pc, void *stream);
 
 #ifdef __cplusplus
 }

@@ -272,7 +272,7 @@ class AdamEMGPU:
 
 
 def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_iter=10000,
-               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None, noise=None):
+               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None, noise=None, labels=None):
     """em_adam_gpu.py:281-356 _optimize_parameters_gpu_once.
 
     Returns (P_best, Q_best).  If ``history`` is a list, (iteration, logL)
@@ -281,7 +281,10 @@ def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, 
     ``noise = (seed, amp)`` multiplies every EM target by (1 + amp z),
     z ~ N(0, 1): with amp ~ 1e-16 it models the rounding differences of
     another FP64 summation order, to measure how far the trajectory moves
-    under them (the chaos floor a GPU run is judged against).
+    under them (the chaos floor a GPU run is compared against).  With
+    ``labels`` (y, one int per sample, 0 = unlabelled) it is supervised.py:
+    342-462 optimize_supervised_gpu: the labelled rows of Q are snapped
+    after every Adam step (:404-406).
     """
     P = _f64(P).copy()
     Q = _f64(Q).copy()
@@ -297,6 +300,8 @@ def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, 
     opt = AdamEMGPU(M, N, K, lr, beta1, beta2, reg_adam)
     PE, QE = opt.run_em_step(Gp, P, Q, M)
     opt.step(P, Q, PE, QE)
+    if labels is not None:
+        snap_q(Q, labels, K)
     L_best = loglik_gpu(Gp, P, Q, M)
     if history is not None:
         history.append((0, L_best))
@@ -305,6 +310,8 @@ def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, 
     for it in range(max_iter):
         PE, QE = perturb(*opt.run_em_step(Gp, P, Q, M))
         opt.step(P, Q, PE, QE)
+        if labels is not None:
+            snap_q(Q, labels, K)
         if (it + 1) % check == 0:
             L = loglik_gpu(Gp, P, Q, M)
             if history is not None:
@@ -322,6 +329,62 @@ def adamem_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, 
                         break
                     wait = 0
     return P_best, Q_best
+
+
+def projection_gpu(Gp, P, Q, M, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_iter=10000, check=5,
+                   lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None, noise=None):
+    """adamixture/src/projection.py:178-321 optimize_projection_gpu: P fixed,
+    Adam-EM on Q only (the P-side EM output is discarded).  The reference's
+    third priming EM step (:300) does not change any state and is omitted.
+    ``noise`` as in adamem_gpu (applied to the EM targets of the loop)."""
+    P = _f64(P)
+    Q = _f64(Q).copy()
+    N, K = Q.shape
+    mQ = np.zeros((N, K))
+    vQ = np.zeros((N, K))
+    t = 0.0
+    dummy = np.zeros((1, K))
+    rng = None if noise is None else np.random.default_rng(noise[0])
+    primed = False
+
+    def em_q():
+        A, B, T, qb = em_stats_gpu(Gp, P, Q, M)
+        QE = em_final_gpu(P, Q, A, B, T, qb)[1]
+        if rng is not None and primed:
+            QE = QE * (1.0 + noise[1] * rng.standard_normal(QE.shape))
+        return QE
+
+    def q_step(QE, lr_now):
+        nonlocal t
+        t += 1.0
+        lib().oc_adam_step_gpu(_p(dummy), _p(Q), _p(dummy), _p(_f64(QE)), _p(dummy), _p(dummy), _p(mQ), _p(vQ),
+                               0, N, K, t, lr_now, beta1, beta2, reg_adam)
+
+    q_step(em_q(), lr)
+    primed = True
+    L_best = loglik_gpu(Gp, P, Q, M)
+    if history is not None:
+        history.append((0, L_best))
+    Q_best = Q.copy()
+    wait = 0
+    for it in range(max_iter):
+        q_step(em_q(), lr)
+        if (it + 1) % check == 0:
+            L = loglik_gpu(Gp, P, Q, M)
+            if history is not None:
+                history.append((it + 1, L))
+            if L > L_best + tol:
+                L_best = L
+                Q_best[:] = Q
+                wait = 0
+            else:
+                wait += 1
+                if wait >= patience:
+                    lr = max(lr * lr_decay, min_lr)
+                    if lr <= min_lr:
+                        break
+                    wait = 0
+    return Q_best
 
 
 # --------------------------------------------------------------------------
@@ -601,11 +664,15 @@ def zal_extrapolate(U: np.ndarray, V: np.ndarray, x: np.ndarray, x_next: np.ndar
 
 
 def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None, history=None,
-         gpu_semantics=False):
+         gpu_semantics=False, labels=None):
     """br_qn.py:73-205 optimize_original (CPU semantics, packed G), or with
     ``gpu_semantics`` br_qn_gpu.py:217-382, which maps the quasi-Newton point
     by clamp + row renormalisation (_mapPQ_gpu, :47-61) instead of the box
-    clip + simplex projection of br_qn.py:159-160."""
+    clip + simplex projection of br_qn.py:159-160.  With ``labels`` (and
+    gpu_semantics) it is supervised.py:602-781 optimize_supervised_original_gpu:
+    labelled Q rows snapped after each Q solve (:689, :697) and after the
+    quasi-Newton map (:741)."""
+    snap = (lambda X: snap_q(X, labels, X.shape[1])) if labels is not None else (lambda X: None)
     P = _f64(P).copy()
     Q = _f64(Q).copy()
     N, K = Q.shape
@@ -620,10 +687,12 @@ def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None
     for it in range(1, max_iter + 1):
         H, z = grad_hess_q(Gp, Q, P, M)
         Qn = sqp_q(H, z, Q, vkk)
+        snap(Qn)
         H, z = grad_hess_p(Gp, Qn, P, M)
         Pn = sqp_p(H, z, P)
         H, z = grad_hess_q(Gp, Qn, Pn, M)
         Qn2 = sqp_q(H, z, Qn, vkk)
+        snap(Qn2)
         H, z = grad_hess_p(Gp, Qn2, Pn, M)
         Pn2 = sqp_p(H, z, Pn)
         x = np.concatenate([P.ravel(), Q.ravel()])
@@ -642,6 +711,7 @@ def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None
         else:
             for i in range(N):
                 Qq[i] = project_simplex(Qq[i])
+        snap(Qq)
         llq = loglik_gpu(Gp, Pq, Qq, M)
         if llq > ll_prev:
             P, Q, ll_new = Pq, Qq, llq
@@ -664,6 +734,210 @@ def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None
     return P_best, Q_best
 
 
+def projection_brqn_gpu(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, history=None):
+    """adamixture/src/projection.py:505-659 optimize_projection_original_gpu:
+    P fixed; per iteration two SQP sweeps on Q, ZAL extrapolation over the
+    flattened N K vector (:589-617), clamp + row renormalisation (:620-621),
+    acceptance against the previous iteration's log-likelihood (:624-630)
+    and its stopping rules (:632-656)."""
+    P = _f64(P)
+    Q = _f64(Q).copy()
+    N, K = Q.shape
+    vkk = v_kk(K)
+    U = np.zeros((N * K, Q_hist))
+    V = np.zeros((N * K, Q_hist))
+    ll_prev = -np.inf
+    ll_best = -np.inf
+    wait = 0
+    Q_best = Q.copy()
+    for it in range(1, max_iter + 1):
+        H, z = grad_hess_q(Gp, Q, P, M)
+        Q1 = sqp_q(H, z, Q, vkk)
+        H, z = grad_hess_q(Gp, Q1, P, M)
+        Q2 = sqp_q(H, z, Q1, vkk)
+        x, x1, x2 = Q.ravel(), Q1.ravel(), Q2.ravel()
+        col = (it - 1) % Q_hist
+        U[:, col] = x1 - x
+        V[:, col] = x2 - x1
+        nc = min(it, Q_hist)
+        Us, Vs = U[:, :nc], V[:, :nc]
+        lhs = Us.T @ (Us - Vs)
+        rhs = Us.T @ (x - x1)
+        try:
+            alpha = np.linalg.solve(lhs, rhs)
+        except np.linalg.LinAlgError:
+            alpha = np.linalg.lstsq(lhs, rhs, rcond=None)[0]
+        Qq = np.clip((x1 - Vs @ alpha).reshape(N, K), 1e-5, 1 - 1e-5)
+        Qq /= Qq.sum(axis=1, keepdims=True)
+        llq = loglik_gpu(Gp, P, Qq, M)
+        if llq > ll_prev:
+            Q, ll_new = Qq, llq
+        else:
+            Q = Q2
+            ll_new = loglik_gpu(Gp, P, Q, M)
+        best_before = ll_best
+        if ll_new > ll_best:
+            ll_best = ll_new
+            Q_best = Q.copy()
+        if best_before != -np.inf and ll_new <= best_before + tol:
+            wait += 1
+        else:
+            wait = 0
+        if history is not None:
+            history.append((it, ll_new))
+        if abs(ll_new - ll_prev) < tol or wait >= patience:
+            break
+        ll_prev = ll_new
+    return Q_best
+
+
+def labels_to_y(raw_labels):
+    """supervised_entry.py:162-188: '-' -> 0, other names -> 1, 2, ... in
+    order of first appearance."""
+    ids = {}
+    y = [0 if lbl == "-" else ids.setdefault(lbl, len(ids) + 1) for lbl in raw_labels]
+    return np.array(y, dtype=np.int32), ids
+
+
+def init_p_supervised(Gp, y, K, M, eps=1e-5):
+    """supervised.py:84-107 init_p_supervised_packed: per-population allele
+    frequencies of the labelled samples from the packed matrix."""
+    Gp = np.asarray(Gp)
+    y = np.asarray(y)
+    P = np.full((M, K), 0.5)
+    rows = np.arange(M)
+    shift = (2 * (rows & 3)).astype(np.uint8)[:, None]
+    for k in range(K):
+        cols = np.flatnonzero(y == k + 1)
+        if cols.size == 0:
+            continue
+        codes = (Gp[(rows >> 2)[:, None], cols[None, :]] >> shift) & np.uint8(3)
+        obs = codes != 3
+        n_obs = obs.sum(axis=1)
+        ok = n_obs > 0
+        s = np.where(obs, codes, 0).astype(np.float64).sum(axis=1)
+        P[ok, k] = s[ok] / (2.0 * n_obs[ok])
+    return P.clip(eps, 1.0 - eps)
+
+
+def init_p_supervised_unpacked(G, y, K, M, eps=1e-5):
+    """supervised.py:49-81 init_p_supervised (unpacked G, missing = 3)."""
+    y = np.asarray(y)
+    P = np.full((M, K), 0.5)
+    for k in range(K):
+        sel = y == k + 1
+        if not sel.any():
+            continue
+        g = G[:, sel].astype(np.float64)
+        miss = g == 3
+        g[miss] = 0.0
+        den = 2.0 * (sel.sum() - miss.sum(axis=1))
+        ok = den > 0
+        P[ok, k] = g.sum(axis=1)[ok] / den[ok]
+    return P.clip(eps, 1.0 - eps)
+
+
+def init_q_supervised(Q, y, K, eps=1e-5):
+    """supervised.py:20-46: labelled rows near-one-hot, unlabelled rows
+    renormalised (in place)."""
+    y = np.asarray(y)
+    snap_q(Q, y, K, eps)
+    un = y == 0
+    Q[un] /= Q[un].sum(axis=1, keepdims=True)
+
+
+def snap_q(Q, y, K, eps=1e-5):
+    """supervised.py:112-131 / :308-339: labelled rows set to eps / max(K-1, 1)
+    with 1 - eps on the labelled population (in place)."""
+    y = np.asarray(y)
+    lab = np.flatnonzero(y > 0)
+    Q[lab] = eps / max(K - 1, 1)
+    Q[lab, y[lab] - 1] = 1.0 - eps
+
+
+# --------------------------------------------------------------------------
+# Cross-validation (adamixture/src/cv.py, br_qn.pyx:92-246)
+# --------------------------------------------------------------------------
+
+SPLITMIX = (0x9E3779B97F4A7C15, 0xBF58476D1CE4E5B9, 0x94D049BB133111EB)
+
+
+def splitmix64(x) -> np.ndarray:
+    """The first output of the splitmix64 generator seeded with x (Steele,
+    Lea & Flood 2014; Vigna's reference code): z = x + 0x9E3779B97F4A7C15,
+    then xor-shift 30 / multiply / xor-shift 27 / multiply / xor-shift 31.
+    cv.py:61-81 _hash_fold_mask hashes flat + seed exactly this way."""
+    with np.errstate(over="ignore"):
+        z = np.asarray(x, dtype=np.uint64) + np.uint64(SPLITMIX[0])
+        z ^= z >> np.uint64(30)
+        z *= np.uint64(SPLITMIX[1])
+        z ^= z >> np.uint64(27)
+        z *= np.uint64(SPLITMIX[2])
+        z ^= z >> np.uint64(31)
+    return z
+
+
+def cv_hash_folds(flat, seed: int, n_folds: int) -> np.ndarray:
+    """Fold id of each flat index (m N + i) on the streaming path."""
+    with np.errstate(over="ignore"):
+        return (splitmix64(np.asarray(flat, dtype=np.uint64) + np.uint64(seed)) % np.uint64(n_folds)).astype(np.int64)
+
+
+def cv_shuffled_entries(G: np.ndarray, seed: int) -> np.ndarray:
+    """cv.py:27-42, :214-219: the observed flat indices as int32, shuffled by
+    numpy's default_rng(seed); fold f is shuffled[f::n_folds]."""
+    flat = np.flatnonzero(G != 3).astype(np.int32)
+    np.random.default_rng(seed).shuffle(flat)
+    return flat
+
+
+def cv_deviance_sum(codes, flat, P, Q, N: int) -> float:
+    """br_qn.pyx:92-149: sum over held-out entries of g log(g/mu) +
+    (2-g) log((2-g)/(2-mu)), mu = 2 Q_i.P_m clamped to [1e-10, 2 - 1e-10],
+    with 0 log 0 = 0.  (The reference sums in an OpenMP reduction.)"""
+    flat = np.asarray(flat, dtype=np.int64)
+    g = np.asarray(codes, dtype=np.float64)
+    row, col = flat // N, flat % N
+    mu = 2.0 * np.einsum("ek,ek->e", _f64(Q)[col], _f64(P)[row])
+    mu = np.clip(mu, 1e-10, 2.0 - 1e-10)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ta = np.where(g > 0, g * np.log(g / mu), 0.0)
+        tb = np.where(2.0 - g > 0, (2.0 - g) * np.log((2.0 - g) / (2.0 - mu)), 0.0)
+    return float(np.sum(ta + tb))
+
+
+def cv_polish(Gp, P, Q, M):
+    """cv.py:146-168 + br_qn.py:50-70 polish_sqp_qn: 5 SQP + ZAL iterations
+    of the CPU optimizer with early stopping disabled (tol 0, patience 6)."""
+    return brqn(Gp, P, Q, M, max_iter=5, tol=0.0, Q_hist=3, patience=6, gpu_semantics=False)
+
+
+def cross_validation(G: np.ndarray, P, Q, seed: int, n_folds: int, streaming: bool | None = None) -> float:
+    """cv.py:171-267 run_cross_validation on an unpacked M x N matrix.
+    ``streaming`` forces the hashed fold path (the reference takes it when
+    M N exceeds the int32 range)."""
+    M, N = G.shape
+    if streaming is None:
+        streaming = M * N > np.iinfo(np.int32).max
+    observed = np.flatnonzero(G != 3)
+    if observed.size == 0:
+        raise ValueError("No non-missing genotypes available for cross-validation.")
+    if streaming:
+        fold_of = cv_hash_folds(observed, seed, n_folds)
+    else:
+        shuffled = cv_shuffled_entries(G, seed)
+    total = 0.0
+    for fold in range(n_folds):
+        held = observed[fold_of == fold] if streaming else shuffled[fold::n_folds]
+        if held.size == 0:
+            continue
+        Gm = G.copy()
+        Gm.ravel()[held] = 3
+        Pc, Qc = cv_polish(pack(Gm), P, Q, M)
+        total += cv_deviance_sum(G.ravel()[held], held, Pc, Qc, N)
+    return total / float(observed.size)
+
+
 PINS = {
     "bed_to_packed / bed_to_unpacked / unpack / pack": "reader fixtures demo_data.G(.packed).expected, exact",
     "flip_packed / mean_packed": "closed form on random tiles (0<->2, padding zeroed) and demo fixture",
@@ -680,4 +954,18 @@ PINS = {
     "qp_sweep / sqp_q / sqp_p / project_simplex": "KKT, brute-force active sets, sort-threshold projection",
     "grad_hess_q / grad_hess_p": "finite differences of loglik_gpu",
     "brqn / zal_extrapolate": "outputs/cpu/brqn/demo_run.brqn.* from the als fixtures",
+    "projection_gpu": "first step equals the Q half of adamem_gpu's priming step; from a random start it "
+                      "reaches the maximiser of the (concave in Q) log-likelihood at the fixture P, i.e. the "
+                      "fixture Q of outputs/gpu/adamem, and agrees with projection_brqn_gpu",
+    "init_p_supervised": "equals init_p_supervised_unpacked (the reference's other restatement) bit for bit "
+                         "on the reader fixture; brute-force loop on a slice",
+    "snap_q / init_q_supervised / labels_to_y": "closed form",
+    "adamem_gpu(labels) / brqn(labels)": "no labels -> the unsupervised path bit for bit; all labelled -> P "
+                                         "satisfies the KKT conditions of max_P logL(P, Q fixed)",
+    "splitmix64 / cv_hash_folds": "splitmix64's published first outputs for seeds 0, C1, 2 C1; folds partition "
+                                  "the observed entries",
+    "cv_deviance_sum": "sum of deviances = -logL(held-out entries) - 2 ln 2 x #heterozygous (logL pinned above); "
+                       "zero for a perfect fit",
+    "cv_polish / cross_validation": "brqn (pinned by the CPU fixture) for exactly 5 iterations; composition",
+    "projection_brqn_gpu": "same concave-maximiser agreement; SQP steps pinned through sqp_q / grad_hess_q",
 }

@@ -47,13 +47,11 @@ def test_sweep_graph_matches_eager(cuda, precision):
         La = np.array([L for _, L in a[K]["checks"]])
         Lb = np.array([L for _, L in b[K]["checks"]])
         print(f"K={K}: {len(La)} checks, max rel dL {np.max(np.abs(La - Lb) / np.abs(Lb)):.2e}", flush=True)
-        np.testing.assert_allclose(La, Lb, rtol=1e-9)
-        # FP64 accumulation reproduces the eager run exactly here; the FP32
-        # accumulators of the f32/bf16 forms add in a different order from
-        # run to run, which 40 Adam steps amplify to ~1e-4..1e-3 in P
-        tol = 1e-12 if precision == "f64" else 3e-3
-        np.testing.assert_allclose(a[K]["P"].cpu().numpy(), b[K]["P"].cpu().numpy(), atol=tol)
-        np.testing.assert_allclose(a[K]["Q"].cpu().numpy(), b[K]["Q"].cpu().numpy(), atol=tol)
+        # the passes sum their partial slices in a fixed order: replaying the
+        # graph reproduces the eager run bit for bit
+        np.testing.assert_array_equal(La, Lb)
+        np.testing.assert_array_equal(a[K]["P"].cpu().numpy(), b[K]["P"].cpu().numpy())
+        np.testing.assert_array_equal(a[K]["Q"].cpu().numpy(), b[K]["Q"].cpu().numpy())
 
 
 def test_sweep_matches_single_k_optimizer(cuda):

@@ -202,3 +202,21 @@ def test_em_pass_bf16_error_vs_oracle(cuda, M, N, K, miss):
         errs.append(np.abs(g64 - want).max() / np.abs(want).max())
         np.testing.assert_allclose(g64, want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
     print(f"bf16 M={M} N={N} K={K}: max rel err A {errs[0]:.2e} B {errs[1]:.2e} T {errs[2]:.2e}")
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
+def test_em_pass_is_deterministic(cuda, precision):
+    """T, A and B are reduced through partial slices summed in a fixed order,
+    so two passes over the same input give bit-identical statistics."""
+    from adamixture_amd.em_adam_gpu import EMAdamOptimizer
+    from adamixture_amd.synth import Workload, make_genotypes
+    w = Workload("det", 3000, 20000, 6, "bn", 0.3, 0.01, fst=0.1, seed=77)
+    G, P0, Q0 = make_genotypes(w, cuda)
+    P = torch.from_numpy(np.clip(P0 * 0.9 + 0.05, 1e-5, 1 - 1e-5)).to(cuda)
+    Q = torch.from_numpy(Q0).to(cuda)
+    opt = EMAdamOptimizer(P.shape, Q.shape, 0.005, 0.8, 0.88, 1e-8, cuda, precision)
+    out = []
+    for _ in range(2):
+        PE, QE = opt.run_em_step(G, P, Q, G.M)
+        out.append((PE.clone(), QE.clone()))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])

@@ -57,3 +57,19 @@ def test_write_outputs_admixture_text_format(tmp_path):
     assert lines[0] == "2.500000000000000000e-01 7.500000000000000000e-01"
     np.testing.assert_array_equal(np.loadtxt(tmp_path / "run.2.Q"), Q)
     np.testing.assert_array_equal(np.loadtxt(tmp_path / "run.2.P", ndmin=2), P)
+
+
+def test_supervised_label_host_logic(tmp_path):
+    from adamixture_amd import supervised as S
+    with pytest.raises(ValueError):
+        S._check_labels(np.array([0, 1, 4]), 3, 3)
+    with pytest.raises(ValueError):
+        S._check_labels(np.array([0, 1]), 3, 3)
+    (tmp_path / "pop.txt").write_text("x\n-\n\ny\nx\n")
+    y, ids = S.labels_to_y(S.parse_labels_file(tmp_path / "pop.txt"))
+    assert y.tolist() == [1, 0, 2, 1] and ids == {"x": 1, "y": 2} and y.dtype == np.int32
+    Q = np.random.default_rng(0).random((4, 2))
+    S.init_q_supervised(Q, y, 2)
+    np.testing.assert_array_equal(Q[[0, 3]], [[1 - 1e-5, 1e-5], [1 - 1e-5, 1e-5]])
+    np.testing.assert_array_equal(Q[2], [1e-5, 1 - 1e-5])
+    np.testing.assert_allclose(Q.sum(axis=1), 1.0, rtol=1e-15)

@@ -362,6 +362,146 @@ def test_brqn_matches_reference_fixture(fixtures):
     np.testing.assert_allclose(Q, fixtures["cpu_brqn_brqn_Q"], atol=1e-8)
 
 
+# ------------------------------------------------------------ projection ----
+
+def _kkt_residual(Gp, P, Q, M):
+    """Relative KKT residuals of max_Q logL(P, Q) on the box-clamped simplex:
+    with z the (positive) Q gradient, interior entries share one multiplier
+    per sample and entries at the lower bound may not exceed it."""
+    _, z = O.grad_hess_q(Gp, Q, P, M)
+    inter = Q > 2e-5
+    lam = np.array([z[i][inter[i]].mean() for i in range(Q.shape[0])])
+    scale = np.abs(z).mean()
+    r_int = np.max(np.abs(np.where(inter, z - lam[:, None], 0.0))) / scale
+    r_bnd = np.max(np.where(~inter, z - lam[:, None], -np.inf)) / scale
+    return r_int, r_bnd
+
+
+def test_projection_priming_is_the_q_half_of_adamem(fixtures):
+    """With P fixed the first Adam step on Q sees the same EM target as the
+    joint Adam-EM priming step (em_adam_gpu.py:311-322 vs projection.py:
+    293-300), so both must give the same Q bit for bit."""
+    Gp, M = fixtures["reader_G_packed"], fixtures["reader_G"].shape[0]
+    P0, Q0 = fixtures["gpu_adamem_als_P"], fixtures["gpu_adamem_als_Q"]
+    _, Qj = O.adamem_gpu(Gp, P0, Q0, M, max_iter=0)
+    Qp = O.projection_gpu(Gp, P0, Q0, M, max_iter=0)
+    np.testing.assert_array_equal(Qp, Qj)
+
+
+def test_projection_reaches_the_concave_maximiser(fixtures):
+    """logL is concave in Q for fixed P, so from a random start both
+    projection algorithms must reach the same maximiser: KKT conditions hold,
+    the two agree, and at the fixture P of the reference's own converged GPU
+    run (outputs/gpu/brqn) they reproduce its fixture Q."""
+    Gp, M = fixtures["reader_G_packed"], fixtures["reader_G"].shape[0]
+    P, Qs = fixtures["gpu_brqn_brqn_P"], fixtures["gpu_brqn_brqn_Q"]
+    rng = np.random.default_rng(42)
+    Q0 = rng.random(Qs.shape)
+    Q0 /= Q0.sum(axis=1, keepdims=True)
+    ha, hb = [], []
+    Qa = O.projection_gpu(Gp, P, Q0, M, history=ha)
+    Qb = O.projection_brqn_gpu(Gp, P, Q0, M, history=hb)
+    r_int, r_bnd = _kkt_residual(Gp, P, Qb, M)
+    assert r_int < 1e-5 and r_bnd < 1e-5
+    r_int, r_bnd = _kkt_residual(Gp, P, Qa, M)
+    assert r_int < 5e-4 and r_bnd < 5e-4                 # Adam stops at tol 0.1 in logL
+    assert np.abs(Qa - Qb).max() < 2e-3
+    assert np.abs(Qb - Qs).max() < 1e-3
+    Ls = O.loglik_gpu(Gp, P, Qs, M)
+    assert hb[-1][1] >= Ls - 1e-3 and ha[-1][1] >= Ls - 0.01
+    assert ha[-1][1] > ha[0][1] and len(ha) > 3 and len(hb) > 2
+    # the KKT check separates: a maximiser paired with a column-swapped P
+    # (a transposed-operand slip) or the random start fails it by far
+    assert _kkt_residual(Gp, P[:, ::-1].copy(), Qb, M)[0] > 1e-3
+    assert _kkt_residual(Gp, P, Q0, M)[0] > 1e-3
+
+
+# ------------------------------------------------------------ supervised ----
+
+def _demo_labels(fixtures, frac=0.9):
+    """Synthetic labels for the demo samples (the reference ships none): the
+    fixture Q's argmax where it exceeds ``frac``, else unlabelled."""
+    Qf = fixtures["gpu_adamem_adamem_Q"]
+    return np.where(Qf.max(axis=1) > frac, Qf.argmax(axis=1) + 1, 0).astype(np.int32)
+
+
+def test_init_p_supervised_packed_equals_unpacked_and_bruteforce(fixtures):
+    G, Gp = fixtures["reader_G"], fixtures["reader_G_packed"]
+    M = G.shape[0]
+    K = 8                                             # population 8 has no labelled sample -> 0.5
+    y = _demo_labels(fixtures)
+    P = O.init_p_supervised(Gp, y, K, M)
+    np.testing.assert_array_equal(P, O.init_p_supervised_unpacked(G, y, K, M))
+    assert np.all(P[:, 7] == 0.5)
+    for m in range(0, M, 397):
+        for k in range(7):
+            obs = [int(G[m, i]) for i in range(G.shape[1]) if y[i] == k + 1 and G[m, i] != 3]
+            want = sum(obs) / (2.0 * len(obs)) if obs else 0.5
+            assert P[m, k] == min(max(want, 1e-5), 1 - 1e-5)
+
+
+def test_snap_init_q_and_labels_closed_form():
+    y, ids = O.labels_to_y(["a", "-", "b", "a", "-", "c"])
+    assert y.tolist() == [1, 0, 2, 1, 0, 3] and ids == {"a": 1, "b": 2, "c": 3}
+    Q = np.random.default_rng(0).random((6, 3))
+    O.init_q_supervised(Q, y, 3)
+    np.testing.assert_array_equal(Q[0], [1 - 1e-5, 5e-6, 5e-6])
+    np.testing.assert_array_equal(Q[5], [5e-6, 5e-6, 1 - 1e-5])
+    np.testing.assert_allclose(Q.sum(axis=1), 1.0, rtol=1e-15)
+
+
+def test_supervised_reduces_to_unsupervised_without_labels(fixtures):
+    Gp, M = fixtures["reader_G_packed"], fixtures["reader_G"].shape[0]
+    P0, Q0 = fixtures["gpu_adamem_als_P"], fixtures["gpu_adamem_als_Q"]
+    none = np.zeros(Q0.shape[0], dtype=np.int32)
+    for a, b in zip(O.adamem_gpu(Gp, P0, Q0, M, labels=none, max_iter=40), O.adamem_gpu(Gp, P0, Q0, M, max_iter=40)):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(O.brqn(Gp, P0, Q0, M, labels=none, gpu_semantics=True, max_iter=4),
+                    O.brqn(Gp, P0, Q0, M, gpu_semantics=True, max_iter=4)):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_supervised_all_labelled_maximises_over_p(fixtures):
+    """Every sample labelled: Q never leaves its snapped rows, and P must be
+    the maximiser of logL(., Q) (concave in P for fixed Q) -- the KKT
+    conditions of the box hold for the SQP result, the Adam result is close
+    to it, and both start from the labelled frequencies."""
+    Gp, M = fixtures["reader_G_packed"], fixtures["reader_G"].shape[0]
+    N, K = fixtures["gpu_adamem_adamem_Q"].shape
+    y = (fixtures["gpu_adamem_adamem_Q"].argmax(axis=1) + 1).astype(np.int32)
+    P0 = O.init_p_supervised(Gp, y, K, M)
+    Q0 = np.random.default_rng(1).random((N, K))
+    O.init_q_supervised(Q0, y, K)
+    Pa, Qa = O.adamem_gpu(Gp, P0, Q0, M, labels=y)
+    Pb, Qb = O.brqn(Gp, P0, Q0, M, labels=y, gpu_semantics=True)
+    np.testing.assert_array_equal(Qa, Q0)
+    np.testing.assert_array_equal(Qb, Q0)
+    _, z = O.grad_hess_p(Gp, Qb, Pb, M)
+    sc = np.abs(z).mean()
+    inter = (Pb > 2e-5) & (Pb < 1 - 2e-5)
+    assert np.abs(np.where(inter, z, 0.0)).max() / sc < 1e-6
+    assert np.where(Pb <= 2e-5, z, -np.inf).max() <= 0 and np.where(Pb >= 1 - 2e-5, -z, -np.inf).max() <= 0
+    assert np.abs(Pa - Pb).max() < 2e-3
+    # the check separates: the starting P is off the maximiser
+    _, z0 = O.grad_hess_p(Gp, Qb, P0, M)
+    assert np.abs(np.where(inter, z0, 0.0)).max() / sc > 1e-4
+
+
+def test_supervised_partial_labels_keep_snapped_rows(fixtures):
+    Gp, M = fixtures["reader_G_packed"], fixtures["reader_G"].shape[0]
+    N, K = fixtures["gpu_adamem_adamem_Q"].shape
+    y = _demo_labels(fixtures)
+    P0 = O.init_p_supervised(Gp, y, K, M)
+    Q0 = np.random.default_rng(1).random((N, K))
+    O.init_q_supervised(Q0, y, K)
+    h = []
+    P, Q = O.adamem_gpu(Gp, P0, Q0, M, labels=y, history=h)
+    lab = y > 0
+    np.testing.assert_array_equal(Q[lab], Q0[lab])
+    assert not np.allclose(Q[~lab], Q0[~lab])
+    assert h[-1][1] > h[0][1]
+
+
 # ------------------------------------------------------- learned warm start ----
 
 def _mlp_case(seed=3, rows=7):
