@@ -61,13 +61,22 @@ def _loglik(G: PackedGenotypes, P, Q) -> float:
     return em_pass(G, P, Q, loglik=True, stats=False).item()
 
 
+def _map_cpu(P: torch.Tensor, Q: torch.Tensor) -> None:
+    """br_qn.py:159-160 (the CPU optimizer's quasi-Newton map): P clipped to
+    the box, each Q row projected onto the simplex with pseudocount 1e-5."""
+    torch.clamp_(P, 1e-5, 1.0 - 1e-5)
+    _lib.call("adx_project_simplex", Q.data_ptr(), Q.shape[0], Q.shape[1], 1e-5, _lib.stream_handle(Q.device))
+
+
 def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patience, device, chunk_size,
-                                threads_per_block, on_iter=None, snap=None):
+                                threads_per_block, on_iter=None, snap=None, qn_map="gpu"):
     """br_qn_gpu.py:217-382.  ``snap(Q)``, if given, is applied to every Q
     the iteration produces -- after each SQP Q solve and after the
     quasi-Newton map -- as the supervised mode does (supervised.py:689, 697,
-    741)."""
+    741).  ``qn_map="cpu"`` maps the quasi-Newton point as the CPU optimizer
+    does (br_qn.py:159-160), which the cross-validation polish runs."""
     snap = snap or (lambda X: None)
+    map_pq = _mapPQ_gpu if qn_map == "gpu" else _map_cpu
     dev = torch.device(device)
     G = as_packed(G, M, dev)
     vkk = v_kk(K, dev)
@@ -109,7 +118,7 @@ def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patienc
         x_qn = x1 - Vs @ alpha
         P_qn = x_qn[:M * K].view(M, K).clone()
         Q_qn = x_qn[M * K:].view(N, K).clone()
-        _mapPQ_gpu(P_qn, Q_qn)
+        map_pq(P_qn, Q_qn)
         snap(Q_qn)
         ll_qn = _loglik(G, P_qn, Q_qn)
         if ll_qn > ll_prev:

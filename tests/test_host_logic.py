@@ -73,3 +73,11 @@ def test_supervised_label_host_logic(tmp_path):
     np.testing.assert_array_equal(Q[[0, 3]], [[1 - 1e-5, 1e-5], [1 - 1e-5, 1e-5]])
     np.testing.assert_array_equal(Q[2], [1e-5, 1 - 1e-5])
     np.testing.assert_allclose(Q.sum(axis=1), 1.0, rtol=1e-15)
+
+
+def test_cv_pack_host_matches_oracle_pack():
+    import oracle as O
+    from adamixture_amd.cv import pack_host
+    for M, N in ((1, 3), (5, 7), (64, 33)):
+        G = np.random.default_rng(M).integers(0, 4, size=(M, N), dtype=np.uint8)
+        np.testing.assert_array_equal(pack_host(G), O.pack(G))

@@ -502,6 +502,63 @@ def test_supervised_partial_labels_keep_snapped_rows(fixtures):
     assert h[-1][1] > h[0][1]
 
 
+# ------------------------------------------------------- cross-validation ----
+
+def test_splitmix64_published_outputs():
+    """cv.py's fold hash is splitmix64: its first three outputs from state 0
+    are the generator's published test values."""
+    C1 = 0x9E3779B97F4A7C15
+    got = [int(O.splitmix64(x)) for x in (0, C1, (2 * C1) % 2**64)]
+    assert got == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_cv_folds_partition_observed_entries(fixtures):
+    G = fixtures["reader_G"]
+    obs = np.flatnonzero(G != 3)
+    fold = O.cv_hash_folds(obs, 42, 5)
+    counts = np.bincount(fold, minlength=5)
+    assert counts.sum() == obs.size and fold.min() >= 0 and fold.max() < 5
+    assert np.all(np.abs(counts - obs.size / 5) < 5 * np.sqrt(obs.size / 5))
+    assert not np.array_equal(fold, O.cv_hash_folds(obs, 43, 5))          # the seed changes the folds
+    sh = O.cv_shuffled_entries(G, 42)
+    parts = [sh[f::5] for f in range(5)]
+    assert np.array_equal(np.sort(np.concatenate(parts)), obs) and sh.dtype == np.int32
+
+
+def test_cv_deviance_is_heldout_loglik_identity(fixtures):
+    """sum of g log(g/mu) + (2-g) log((2-g)/(2-mu)) with mu = 2h equals
+    -logL(held-out entries) - 2 ln 2 x #heterozygous held-out entries
+    (logL pinned above); a brute-force loop checks single entries."""
+    G = fixtures["reader_G"]
+    M, N = G.shape
+    P, Q = fixtures["cpu_brqn_brqn_P"], fixtures["cpu_brqn_brqn_Q"]
+    obs = np.flatnonzero(G != 3)
+    held = obs[O.cv_hash_folds(obs, 7, 5) == 2]
+    codes = G.ravel()[held]
+    dev = O.cv_deviance_sum(codes, held, P, Q, N)
+    Gh = np.full_like(G, 3)
+    Gh.ravel()[held] = codes
+    ll = O.loglik_gpu(O.pack(Gh), P, Q, M)
+    assert abs(dev - (-ll - 2 * np.log(2) * np.count_nonzero(codes == 1))) < 1e-9 * abs(dev)
+    import math
+    for e in held[:50]:
+        m, i = divmod(int(e), N)
+        g = float(G[m, i])
+        mu = min(max(2 * sum(Q[i, k] * P[m, k] for k in range(Q.shape[1])), 1e-10), 2 - 1e-10)
+        want = (g * math.log(g / mu) if g > 0 else 0.0) + ((2 - g) * math.log((2 - g) / (2 - mu)) if g < 2 else 0.0)
+        assert abs(O.cv_deviance_sum([G[m, i]], [e], P, Q, N) - want) < 1e-12 * max(1.0, abs(want))
+
+
+def test_cv_polish_runs_exactly_five_iterations(fixtures):
+    """br_qn.py:50-70 with cv.py's n_iters = 5: tol 0 and patience 6 disable
+    every early stop (as the reference's own test_cv.py asserts of the call)."""
+    Gp, M = fixtures["reader_G_packed"], fixtures["reader_G"].shape[0]
+    h = []
+    O.brqn(Gp, fixtures["cpu_brqn_brqn_P"], fixtures["cpu_brqn_brqn_Q"], M, max_iter=5, tol=0.0, Q_hist=3,
+           patience=6, history=h)
+    assert [it for it, _ in h] == [1, 2, 3, 4, 5]
+
+
 # ------------------------------------------------------- learned warm start ----
 
 def _mlp_case(seed=3, rows=7):
