@@ -90,11 +90,56 @@ def polish_gpu(Gm: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor):
                                        POLISH_ITERS + 1, Gm.device, 0, 0, qn_map="cpu")
 
 
+class FoldScorer:
+    """Per-fold work of run_cross_validation on one device: mask, polish,
+    deviance.  ``fold_sum(fold)`` returns the fold's deviance sum (0.0 for an
+    empty fold)."""
+
+    def __init__(self, Gp: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, n_folds: int, seed: int,
+                 streaming: bool):
+        self.Gp, self.P, self.Q = Gp, P, Q
+        self.n_folds, self.seed, self.streaming = n_folds, seed, streaming
+        dev = Gp.device
+        self.st = _lib.stream_handle(dev)
+        self.work = torch.empty(_lib.ADX_CV_BLOCKS, dtype=torch.float64, device=dev)
+        self.out = torch.empty(1, dtype=torch.float64, device=dev)
+        self.Gm = PackedGenotypes(torch.empty_like(Gp.data), Gp.M, Gp.N)
+        if not streaming:
+            flat = observed_flat_host(Gp).astype(np.int32)
+            np.random.default_rng(seed).shuffle(flat)           # cv.py:27-42
+            self.n_flat = flat.size
+            self.flat_d = torch.from_numpy(flat).to(dev)
+
+    def fold_sum(self, fold: int) -> float:
+        Gp, Gm, M, N = self.Gp, self.Gm, self.Gp.M, self.Gp.N
+        seed = ctypes.c_uint64(self.seed).value
+        if self.streaming:
+            _lib.call("adx_cv_mask_hash", Gp.data.data_ptr(), Gm.data.data_ptr(), Gp.ldg, M, N, seed, self.n_folds,
+                      fold, self.st)
+        else:
+            n = len(range(fold, self.n_flat, self.n_folds))
+            if n == 0:
+                return 0.0
+            Gm.data.copy_(Gp.data)
+            _lib.call("adx_cv_mask_list", Gm.data.data_ptr(), Gm.ldg, M, N, self.flat_d.data_ptr(), 0, fold,
+                      self.n_folds, n, self.st)
+        Gm._qbat = None                                    # contents changed
+        P_cv, Q_cv = polish_gpu(Gm, self.P, self.Q)
+        K = self.P.shape[1]
+        if self.streaming:
+            _lib.call("adx_cv_deviance_hash", Gp.data.data_ptr(), Gp.ldg, M, N, seed, self.n_folds, fold,
+                      P_cv.data_ptr(), Q_cv.data_ptr(), K, self.work.data_ptr(), self.out.data_ptr(), self.st)
+        else:
+            _lib.call("adx_cv_deviance_list", Gp.data.data_ptr(), Gp.ldg, M, N, self.flat_d.data_ptr(), 0, fold,
+                      self.n_folds, n, P_cv.data_ptr(), Q_cv.data_ptr(), K, self.work.data_ptr(),
+                      self.out.data_ptr(), self.st)
+        return self.out.item()
+
+
 def run_cross_validation(args, G, N: int, M: int, K: int, P_global, Q_global, *, device=None,
                          streaming: bool | None = None) -> float:
     """cv.py:171-267.  ``args`` needs ``cv`` (number of folds) and ``seed``."""
     n_folds = int(args.cv)
-    seed = int(args.seed)
     dev = torch.device(device) if device is not None else (
         P_global.device if isinstance(P_global, torch.Tensor) and P_global.is_cuda else torch.device("cuda"))
     Gp = _packed_input(G, M, N, dev)
@@ -107,35 +152,6 @@ def run_cross_validation(args, G, N: int, M: int, K: int, P_global, Q_global, *,
     n_observed = _observed_count(Gp)
     if n_observed == 0:
         raise ValueError("No non-missing genotypes available for cross-validation.")
-    st = _lib.stream_handle(dev)
-    work = torch.empty(_lib.ADX_CV_BLOCKS, dtype=torch.float64, device=dev)
-    fold_sum = torch.empty(1, dtype=torch.float64, device=dev)
-    Gm = PackedGenotypes(torch.empty_like(Gp.data), M, N)
-
-    if not streaming:
-        flat = observed_flat_host(Gp).astype(np.int32)
-        np.random.default_rng(seed).shuffle(flat)           # cv.py:27-42
-        flat_d = torch.from_numpy(flat).to(dev)
-
-    cv_sum = 0.0
-    for fold in range(n_folds):
-        if streaming:
-            _lib.call("adx_cv_mask_hash", Gp.data.data_ptr(), Gm.data.data_ptr(), Gp.ldg, M, N,
-                      ctypes.c_uint64(seed).value, n_folds, fold, st)
-        else:
-            n = len(range(fold, flat.size, n_folds))
-            if n == 0:
-                continue
-            Gm.data.copy_(Gp.data)
-            _lib.call("adx_cv_mask_list", Gm.data.data_ptr(), Gm.ldg, M, N, flat_d.data_ptr(), 0, fold, n_folds, n,
-                      st)
-        Gm._qbat = None                                    # contents changed
-        P_cv, Q_cv = polish_gpu(Gm, P, Q)
-        if streaming:
-            _lib.call("adx_cv_deviance_hash", Gp.data.data_ptr(), Gp.ldg, M, N, ctypes.c_uint64(seed).value,
-                      n_folds, fold, P_cv.data_ptr(), Q_cv.data_ptr(), K, work.data_ptr(), fold_sum.data_ptr(), st)
-        else:
-            _lib.call("adx_cv_deviance_list", Gp.data.data_ptr(), Gp.ldg, M, N, flat_d.data_ptr(), 0, fold,
-                      n_folds, n, P_cv.data_ptr(), Q_cv.data_ptr(), K, work.data_ptr(), fold_sum.data_ptr(), st)
-        cv_sum += fold_sum.item()
+    scorer = FoldScorer(Gp, P, Q, n_folds, int(args.seed), streaming)
+    cv_sum = sum(scorer.fold_sum(fold) for fold in range(n_folds))
     return cv_sum / float(n_observed)

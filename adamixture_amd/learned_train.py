@@ -198,14 +198,19 @@ def first_reach(checks, target: float):
     return None
 
 
+class _Reached(Exception):
+    pass
+
+
 def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_precision: str = "f32",
-         fast_precision: str = "bf16", rel: float = 1e-6) -> dict:
+         fast_precision: str = "bf16", rel: float = 1e-6, extra_plain: tuple = ()) -> dict:
     """Time from the start of initialisation to the first logL check within
     ``rel`` of L*, where L* is the FP64 logL of the converged plain GPU run
     (randomized SVD + ALS start, Adam-EM in ``plain_precision``).  Races:
     plain start in ``plain_precision`` (defines L*), plain start in
     ``fast_precision``, and (with a model) the learned start in
-    ``fast_precision``."""
+    ``fast_precision``.  Each precision in ``extra_plain`` (e.g. "f64", the
+    reference's arithmetic) races the plain start too, stopped at the target."""
     from .svd_gpu import SVD_gpu as _svd
     from .learned_init import learned_warm_start
     dev = torch.device(device)
@@ -250,10 +255,34 @@ def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_p
         P3, Q3, ch_l = run_to_stop(G, P_w, Q_w, K, dev, fast_precision, t4)
         summary(f"learned_{fast_precision}", t_init_l, ch_l, P3, Q3,
                 {"init_logL": ll_w, "used_learned": bool(used)})
+    for prec in extra_plain:
+        checks = []
+
+        def on_check(it, ll, checks=checks, t_origin=_sync_time(dev)):
+            checks.append((it, ll, time.perf_counter() - t_origin))
+            if ll >= target:
+                raise _Reached
+
+        try:
+            converge_plain(G, P_als, Q_als, K, dev, prec, on_check)
+        except _Reached:
+            pass
+        hit = first_reach(checks, target)
+        rec = {"init_s": round(t_init, 3), "reached": hit is not None, "stopped_at_target": True}
+        if hit is not None:
+            rec["iterations_to_target"] = hit[0]
+            rec["seconds_to_target"] = round(t_init + hit[1], 3)
+        out[f"plain_{prec}"] = rec
+        print(f"[race] plain_{prec}: {json.dumps(rec)}", flush=True)
     base = out[f"plain_{plain_precision}"].get("seconds_to_target")
     for name in list(out):
         if isinstance(out[name], dict) and out[name].get("seconds_to_target") and base:
             out[name]["speedup_vs_plain"] = round(base / out[name]["seconds_to_target"], 3)
+    for prec in extra_plain:
+        ref = out[f"plain_{prec}"].get("seconds_to_target")
+        for name in list(out):
+            if isinstance(out[name], dict) and out[name].get("seconds_to_target") and ref:
+                out[name][f"speedup_vs_plain_{prec}"] = round(ref / out[name]["seconds_to_target"], 3)
     return out
 
 

@@ -1,0 +1,114 @@
+"""Timing of the projection, supervised and cross-validation modes on a
+synthetic workload (``python -m adamixture_amd.mode_timing --mode cv``).
+
+Prints one JSON line per mode: wall seconds per iteration (Adam-EM modes,
+iterations after the priming step, logL checks included) or per fold (CV:
+mask + 5 polish iterations + deviance).  Genotypes are generated on the
+device by the counter-based generator of ``synth``.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+import torch
+
+from .synth import WORKLOADS, make_genotypes
+
+ADAM = dict(lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, check=5, lr_decay=0.5, min_lr=1e-4, patience=3,
+            tol_adam=0.1)
+
+
+def _sync(dev) -> float:
+    torch.cuda.synchronize(dev)
+    return time.perf_counter()
+
+
+def _start(w, P0, Q0, seed):
+    rng = np.random.default_rng(seed)
+    P = np.clip(P0 + rng.normal(0.0, 0.02, P0.shape), 1e-5, 1 - 1e-5)
+    Q = rng.random(Q0.shape)
+    return P, Q / Q.sum(axis=1, keepdims=True)
+
+
+def time_projection(G, P0, Q0, w, dev, precision: str, iters: int) -> dict:
+    from .projection import optimize_projection_gpu
+    Pf = torch.from_numpy(np.clip(P0, 1e-5, 1 - 1e-5)).to(dev)
+    _, Q = _start(w, P0, Q0, 1)
+
+    def run(n):
+        Qd = torch.from_numpy(Q).to(dev)
+        t0 = _sync(dev)
+        optimize_projection_gpu(G, Pf, Qd, ADAM["lr"], ADAM["beta1"], ADAM["beta2"], ADAM["reg_adam"], n,
+                                ADAM["check"], w.M, ADAM["lr_decay"], ADAM["min_lr"], ADAM["patience"],
+                                ADAM["tol_adam"], dev, 0, 0, precision=precision)
+        return _sync(dev) - t0
+
+    run(ADAM["check"])                                   # warm-up (code load, autotune-free defaults)
+    t_short, t_long = run(ADAM["check"]), run(ADAM["check"] + iters)
+    return {"mode": "projection_adamem", "precision": precision, "iterations": iters,
+            "s_per_iteration": round((t_long - t_short) / iters, 5)}
+
+
+def time_supervised(G, P0, Q0, w, dev, precision: str, iters: int) -> dict:
+    from .supervised import init_q_supervised, label_freq_gpu, optimize_supervised_gpu
+    rng = np.random.default_rng(3)
+    y = np.where(rng.random(w.N) < 0.5, Q0.argmax(axis=1) + 1, 0).astype(np.int32)
+    t0 = _sync(dev)
+    P = label_freq_gpu(G, y, w.K)
+    t_init = _sync(dev) - t0
+    _, Q = _start(w, P0, Q0, 2)
+    init_q_supervised(Q, y, w.K)
+
+    def run(n):
+        t0 = _sync(dev)
+        optimize_supervised_gpu(G, P.clone(), torch.from_numpy(Q).to(dev), y, ADAM["lr"], ADAM["beta1"],
+                                ADAM["beta2"], ADAM["reg_adam"], n, ADAM["check"], w.M, ADAM["lr_decay"],
+                                ADAM["min_lr"], ADAM["patience"], ADAM["tol_adam"], dev, 0, 0, precision=precision)
+        return _sync(dev) - t0
+
+    run(ADAM["check"])
+    t_short, t_long = run(ADAM["check"]), run(ADAM["check"] + iters)
+    return {"mode": "supervised_adamem", "precision": precision, "iterations": iters,
+            "s_per_iteration": round((t_long - t_short) / iters, 5), "label_freq_s": round(t_init, 5),
+            "labelled": int((y > 0).sum())}
+
+
+def time_cv(G, P0, Q0, w, dev, folds: int, timed_folds: int) -> dict:
+    """Folds 0 .. timed_folds-1 of a ``folds``-fold CV on the hashed path."""
+    from .cv import FoldScorer
+    P, Q = _start(w, P0, Q0, 4)
+    scorer = FoldScorer(G, torch.from_numpy(P).to(dev), torch.from_numpy(Q).to(dev), folds, 42, streaming=True)
+    t0 = _sync(dev)
+    sums = [scorer.fold_sum(f) for f in range(timed_folds)]
+    dt = _sync(dev) - t0
+    return {"mode": "cv", "folds": folds, "timed_folds": timed_folds, "path": "hashed",
+            "s_per_fold": round(dt / timed_folds, 4), "fold_deviance_sums": sums}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--mode", choices=["projection", "supervised", "cv", "all"], default="all")
+    ap.add_argument("--workload", default="biobank", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="bf16", choices=["f64", "f32", "bf16"])
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--folds", type=int, default=5)
+    ap.add_argument("--timed-folds", type=int, default=1)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    w = WORKLOADS[args.workload]
+    G, P0, Q0 = make_genotypes(w, dev)
+    base = {"workload": f"{w.name}: N={w.N} M={w.M} K={w.K}"}
+    if args.mode in ("projection", "all"):
+        print(json.dumps({**base, **time_projection(G, P0, Q0, w, dev, args.precision, args.iters)}), flush=True)
+    if args.mode in ("supervised", "all"):
+        print(json.dumps({**base, **time_supervised(G, P0, Q0, w, dev, args.precision, args.iters)}), flush=True)
+    if args.mode in ("cv", "all"):
+        print(json.dumps({**base, **time_cv(G, P0, Q0, w, dev, args.folds, args.timed_folds)}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -213,14 +213,15 @@ def main():
 def time_to_target(G, w, dev):
     """Seconds from the start of initialisation to the first logL check
     within 1e-6 relative of L* (FP64 logL of the converged plain FP32 run
-    from the randomized-SVD + ALS start): plain FP32, plain BF16 and the
-    learned start + BF16 (when trained weights are present)."""
+    from the randomized-SVD + ALS start): plain FP32, plain BF16, the
+    learned start + BF16 (when trained weights are present), and plain FP64
+    (the reference's arithmetic; stopped once it reaches the target)."""
     from adamixture_amd.genotypes import allele_frequencies
     from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP
     from adamixture_amd.learned_train import race
     model = WarmStartMLP.load(dev) if WEIGHTS_PATH.exists() else None
     f = allele_frequencies(G)
-    return race(G, f, w.K, w.seed, dev, model, plain_precision="f32", fast_precision="bf16")
+    return race(G, f, w.K, w.seed, dev, model, plain_precision="f32", fast_precision="bf16", extra_plain=("f64",))
 
 
 if __name__ == "__main__":
