@@ -65,6 +65,9 @@ SIGNATURES = {
     "adx_cv_deviance_list": (_int, [_p, _i64, _i64, _i64, _p, _int, _i64, _i64, _i64, _p, _p, _int, _p, _p, _p]),
     "adx_cv_deviance_hash": (_int, [_p, _i64, _i64, _i64, _u64, _int, _int, _p, _p, _int, _p, _p, _p]),
     "adx_project_simplex": (_int, [_p, _i64, _int, _dbl, _p]),
+    "adx_vcf_gt": (_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p]),
+    "adx_bcf_gt": (_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p]),
+    "adx_clear_tail_fields": (_int, [_p, _i64, _i64, _i64, _p]),
     "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p]),
     "adx_gemm_nn_f64": (_int, [_p, _i64, _i64, _int, _int, _p, _i64, _dbl, _p, _p, _dbl, _p, _i64, _p]),
     "adx_set_em_tiles": (_int, [_int, _int, _i64]),

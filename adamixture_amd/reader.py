@@ -141,13 +141,25 @@ def read_bed_gpu(path: str | Path, device="cuda", chrom_mode: str = "all", autos
         rows = np.asarray(bed[s:e]) if contiguous else bed[idx[s:e]]
         dbed = torch.from_numpy(np.ascontiguousarray(rows)).to(dev)
         _lib.call("adx_bed_to_packed", dbed.data_ptr(), nb, e - s, N, data.data_ptr(), ldg, s // 4, st)
-    sums = torch.empty(2, dtype=torch.int64, device=dev)     # uint64 counters, < 2^63
-    _lib.call("adx_packed_allele_sum", data.data_ptr(), ldg, M, N, sums.data_ptr(), st)
+    G = PackedGenotypes(data, M, N)
+    flip_gpu_if_needed(G)
+    return G, N, M
+
+
+def flip_gpu_if_needed(G) -> bool:
+    """snp_reader.py:747-757 on the device: flip 0 <-> 2 when the mean allele
+    frequency of the observed genotypes is >= 0.5.  Returns whether it did."""
+    from . import _lib
+    st = _lib.stream_handle(G.device)
+    sums = torch.empty(2, dtype=torch.int64, device=G.device)     # uint64 counters, < 2^63
+    _lib.call("adx_packed_allele_sum", G.data.data_ptr(), G.ldg, G.M, G.N, sums.data_ptr(), st)
     s_all, c_all = (int(x) for x in sums.cpu().tolist())
     if c_all and s_all / c_all >= 0.5:
         log.info("    Flipping genotype encoding (packed).")
-        _lib.call("adx_flip_packed", data.data_ptr(), ldg, M, N, st)
-    return PackedGenotypes(data, M, N), N, M
+        _lib.call("adx_flip_packed", G.data.data_ptr(), G.ldg, G.M, G.N, st)
+        G._qbat = None
+        return True
+    return False
 
 
 def write_bed(base: str | Path, G: np.ndarray) -> None:

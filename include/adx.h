@@ -313,6 +313,30 @@ int adx_label_freq(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, c
  * rows with y[i] <= 0 are unchanged.  y is a device int32 array. */
 int adx_snap_labels(double *Q, int64_t N, int K, const int32_t *y, double eps, void *stream);
 
+/* --- VCF / BCF decoding ---------------------------------------------------- */
+
+/* Replaces the GT parsing of read_vcf_file(_packed)
+ * (adamixture/src/utils_c/cython/snp_reader.pyx:343-463, 554-827): data line
+ * l of `text` (device) spans [line_start[l], line_end[l]) with text[line_end[l]]
+ * == '\n'; its genotypes become SNP m0 + l of G (ceil(M/4) x ldg, ldg a
+ * multiple of 4), whose fields the caller set to 3 (bytes 0xFF) beforehand.
+ * GT must be the first FORMAT subfield, as the reference assumes. */
+int adx_vcf_gt(const uint8_t *text, const int64_t *line_start, const int64_t *line_end, int64_t n_lines, int64_t m0,
+               int64_t N, uint8_t *G, int64_t ldg, void *stream);
+
+/* Replaces the GT decoding of read_bcf_file(_packed) (snp_reader.pyx:
+ * 1003-1108): record r's GT vector starts at buf[gt_off[r]] with
+ * gt_desc[r] = n_values | (bytes per value << 16), or gt_desc[r] < 0 for a
+ * record without a usable GT field (all missing); it becomes SNP m0 + r of G
+ * (fields preset to 3 as above).  Codes are masked to 2 bits as the
+ * reference's packing masks them. */
+int adx_bcf_gt(const uint8_t *buf, const int64_t *gt_off, const int32_t *gt_desc, int64_t n_rec, int64_t m0,
+               int64_t N, uint8_t *G, int64_t ldg, void *stream);
+
+/* Fields of the last byte row beyond SNP M - 1 set to 0 (pack_genotypes,
+ * snp_reader.pyx:123-157, leaves them 0). */
+int adx_clear_tail_fields(uint8_t *G, int64_t ldg, int64_t M, int64_t N, void *stream);
+
 /* --- cross-validation ------------------------------------------------------ */
 
 /* Partial-sum slots the deviance entry points need in `work` (doubles). */

@@ -938,6 +938,253 @@ def cross_validation(G: np.ndarray, P, Q, seed: int, n_folds: int, streaming: bo
     return total / float(observed.size)
 
 
+# --------------------------------------------------------------------------
+# VCF / BCF genotype decoding (snp_reader.pyx:305-463, :829-1108)
+# --------------------------------------------------------------------------
+
+def _vcf_allele(s: bytes, q: int):
+    """snp_reader.pyx:305-341: (allele or 255 for missing, next position);
+    bytes past the end read as NUL."""
+    c = s[q] if q < len(s) else 0
+    if c in (46, 45, 0):
+        return 255, (q + 1 if c != 0 else q)
+    start, val = q, 0
+    while q < len(s) and 48 <= s[q] <= 57:
+        val = (val * 10 + s[q] - 48) & 0xFF
+        q += 1
+    return (255 if q == start else val), q
+
+
+def vcf_gt_code(line: bytes, q: int) -> int:
+    """snp_reader.pyx:343-375 on the sample field starting at line[q]; the
+    line keeps its trailing '\\n' and reads as NUL beyond it."""
+    a1, q = _vcf_allele(line, q)
+    if a1 == 255:
+        return 3
+    sep = line[q] if q < len(line) else 0
+    if sep in (58, 9, 10, 0):
+        return a1 * 2 if a1 <= 1 else 3
+    a2, q = _vcf_allele(line, q + 1)
+    if a2 == 255:
+        return 3
+    t = (a1 + a2) & 0xFF
+    return t if t <= 2 else 3
+
+
+def vcf_to_unpacked(data: bytes, keep=lambda tok: True):
+    """read_vcf_file_packed's parse (snp_reader.pyx:446-462, 710-827) on a
+    decompressed VCF: (M x N codes, N, M); samples a short line lacks are
+    missing.  ``keep(chrom_token)`` filters lines."""
+    N, rows = 0, []
+    for line in data.split(b"\n")[:-1] if data.endswith(b"\n") else data.split(b"\n"):
+        line = line + b"\n"
+        if line.startswith(b"#"):
+            if line.startswith((b"#CHROM", b"#chrom")):
+                N = len(line.rstrip(b"\n").split(b"\t")) - 9
+            continue
+        if not keep(line.split(b"\t", 1)[0].rstrip(b"\n")):
+            continue
+        row = np.full(N, 3, dtype=np.uint8)
+        tabs = [k for k, c in enumerate(line) if c == 9]
+        for i in range(N):
+            q = tabs[8 + i] + 1 if 8 + i < len(tabs) else None
+            if q is not None:
+                row[i] = vcf_gt_code(line, q)
+        rows.append(row)
+    return np.array(rows, dtype=np.uint8).reshape(len(rows), N), N, len(rows)
+
+
+def bcf_to_unpacked(data: bytes, keep=lambda tok: True):
+    """read_bcf_file (snp_reader.pyx:859-1080) on a decompressed BCF: (M x N
+    values, N, M).  Values are a1 + a2 (a1 for one value per sample), 3
+    where an allele is missing; the reference's packed reader keeps the low
+    2 bits of them."""
+    import struct as _st
+    l_text = _st.unpack_from("<I", data, 5)[0]
+    header = data[9:9 + l_text].decode("utf-8", errors="ignore")
+    contigs, samples, gt_idx = {}, [], -1
+    for line in header.splitlines():
+        if line.startswith("##contig=") and "ID=" in line:
+            contigs[len(contigs)] = line.split("ID=")[1].split(",")[0].split(">")[0]
+        elif line.startswith(("#CHROM", "#chrom")):
+            samples = line.split("\t")[9:]
+    for line in header.splitlines():
+        if line.startswith("##FORMAT=") and "ID=GT" in line:
+            if "IDX=" in line:
+                gt_idx = int(line.split("IDX=")[1].split(",")[0].split(">")[0])
+            break
+    N = len(samples)
+
+    def typed_int(pos):
+        b = data[pos]
+        t, n = b & 15, b >> 4
+        pos += 1
+        if n == 15:
+            _, pos = typed_int(pos)
+        fmt = {1: "<b", 2: "<h", 3: "<i"}.get(t)
+        if fmt is None:
+            return 0, pos
+        return _st.unpack_from(fmt, data, pos)[0], pos + _st.calcsize(fmt)
+
+    rows, off = [], 9 + l_text
+    while off + 8 <= len(data):
+        l_shared, l_indiv = _st.unpack_from("<II", data, off)
+        base = off + 8
+        chrom = contigs.get(_st.unpack_from("<i", data, base)[0], "")
+        if keep(chrom.encode()):
+            n_fmt = _st.unpack_from("<I", data, base + 20)[0] >> 24
+            pos, gt = base + l_shared, None
+            for _ in range(n_fmt):
+                key, pos = typed_int(pos)
+                b = data[pos]
+                t, n = b & 15, b >> 4
+                pos += 1
+                if n == 15:
+                    n, pos = typed_int(pos)
+                sz = 1 if t in (1, 7) else (2 if t == 2 else 4)
+                if key == gt_idx or gt_idx < 0:
+                    gt = (pos, n, sz)
+                    break
+                pos += N * n * sz
+            row = np.full(N, 3, dtype=np.int64)
+            if gt is not None and gt[1] >= 1 and gt[0] + N * gt[1] * gt[2] <= len(data):
+                p, n, sz = gt
+                vals = np.frombuffer(data, dtype={1: "<i1", 2: "<i2", 4: "<i4"}[sz], count=N * n, offset=p)
+                vals = vals.reshape(N, n).astype(np.int64)
+                a1 = (vals[:, 0] >> 1) - 1
+                if n >= 2:
+                    a2 = (vals[:, 1] >> 1) - 1
+                    row = np.where((a1 < 0) | (a2 < 0), 3, a1 + a2)
+                else:
+                    row = np.where(a1 < 0, 3, a1)
+            rows.append(row)
+        off = base + l_shared + l_indiv
+    return np.array(rows, dtype=np.int64).reshape(len(rows), N), N, len(rows)
+
+
+# --------------------------------------------------------------------------
+# PGEN hardcall decoding (the reference reads PGEN through pgenlib >= 0.94.1,
+# snp_reader.pyx:1110-1135; restated from the published PLINK 2 .pgen format)
+# --------------------------------------------------------------------------
+
+def _varint(buf: bytes, p: int):
+    v, sh = 0, 0
+    while True:
+        b = buf[p]
+        p += 1
+        v |= (b & 0x7F) << sh
+        sh += 7
+        if b < 0x80:
+            return v, p
+
+
+def pgen_difflist(buf: bytes, p: int, N: int):
+    """A difflist at buf[p]: varint length L; per group of 64 entries the
+    first sample ID (ceil(log256 N) bytes each); L/64 - 1 skip bytes; the L
+    2-bit values (low bits first); then, group by group, the varint gaps to
+    the next sample ID.  Returns (ids, values, end)."""
+    L, p = _varint(buf, p)
+    if L == 0:
+        return [], [], p
+    idb = 1 if N <= 1 << 8 else (2 if N <= 1 << 16 else (3 if N <= 1 << 24 else 4))
+    gc = (L + 63) // 64
+    first = [int.from_bytes(buf[p + idb * g:p + idb * (g + 1)], "little") for g in range(gc)]
+    p += idb * gc + (gc - 1)
+    vals = [(buf[p + k // 4] >> (2 * (k % 4))) & 3 for k in range(L)]
+    p += (L + 3) // 4
+    ids = []
+    for g in range(gc):
+        cur = first[g]
+        ids.append(cur)
+        for _ in range(1, min(64, L - 64 * g)):
+            gap, p = _varint(buf, p)
+            cur += gap
+            ids.append(cur)
+    return ids, vals, p
+
+
+def pgen_header(data: bytes):
+    """Storage mode 0x10 header: (M, N, record starts, record lengths,
+    record types); a variant block of 2^16 variants carries its 4- or 8-bit
+    types and 1-4 byte record lengths (then optional allele counts and
+    provisional-reference flags), blocks in order after the block offsets."""
+    if data[:2] != b"\x6c\x1b" or data[2] != 0x10:
+        raise ValueError("not a variable-width (mode 0x10) .pgen file")
+    M = int.from_bytes(data[3:7], "little")
+    N = int.from_bytes(data[7:11], "little")
+    ctl = data[11]
+    if ctl & 0x30:
+        raise ValueError(".pgen with stored allele counts (multiallelic variants) is not supported")
+    nblk = (M + 65535) // 65536
+    offs = [int.from_bytes(data[12 + 8 * b:20 + 8 * b], "little") for b in range(nblk)]
+    lb = (ctl & 3) + 1
+    wide = (ctl >> 2) & 1
+    pos = 12 + 8 * nblk
+    types, lens = [], []
+    for b in range(nblk):
+        cnt = min(65536, M - 65536 * b)
+        if wide:
+            t = np.frombuffer(data, np.uint8, cnt, pos)
+            pos += cnt
+        else:
+            v = np.frombuffer(data, np.uint8, (cnt + 1) // 2, pos)
+            pos += (cnt + 1) // 2
+            t = np.stack([v & 15, v >> 4], 1).reshape(-1)[:cnt]
+        L = np.frombuffer(data, np.uint8, cnt * lb, pos).reshape(cnt, lb).astype(np.int64)
+        pos += cnt * lb
+        if (ctl >> 6) == 3:
+            pos += (cnt + 7) // 8
+        types.append(t)
+        lens.append((L << (8 * np.arange(lb))).sum(axis=1))
+    types, lens = np.concatenate(types), np.concatenate(lens)
+    starts = np.empty(M, np.int64)
+    for b in range(nblk):
+        s, e = 65536 * b, min(M, 65536 * (b + 1))
+        starts[s:e] = offs[b] + np.concatenate([[0], np.cumsum(lens[s:e])[:-1]])
+    return M, N, starts, lens, types
+
+
+def pgen_to_unpacked(data: bytes) -> tuple[np.ndarray, int, int]:
+    """Hardcalls (ALT allele counts, 3 = missing) of every variant:
+    record type 0 = 2-bit genotype array; 1 = two values chosen by a bit
+    array (first byte: low value = byte >> 2, high = low + (byte & 3)) plus a
+    difflist; 4 = all 0 plus a difflist; 2 = the previous non-LD variant's
+    genotypes patched by a difflist; 3 = the same, then 0 <-> 2 inverted."""
+    M, N, starts, lens, types = pgen_header(data)
+    out = np.zeros((M, N), np.uint8)
+    inv = np.array([2, 1, 0, 3], np.uint8)
+    base = None
+    for m in range(M):
+        t = int(types[m]) & 7
+        p = int(starts[m])
+        if t == 0:
+            v = np.frombuffer(data, np.uint8, (N + 3) // 4, p)
+            row = np.stack([(v >> (2 * k)) & 3 for k in range(4)], 1).reshape(-1)[:N]
+        elif t == 1:
+            lo = data[p] >> 2
+            hi = lo + (data[p] & 3)
+            bits = np.unpackbits(np.frombuffer(data, np.uint8, (N + 7) // 8, p + 1), bitorder="little")[:N]
+            row = np.where(bits == 1, hi, lo).astype(np.uint8)
+            ids, vals, _ = pgen_difflist(data, p + 1 + (N + 7) // 8, N)
+            row[ids] = vals
+        elif t == 4:
+            row = np.zeros(N, np.uint8)
+            ids, vals, _ = pgen_difflist(data, p, N)
+            row[ids] = vals
+        elif t in (2, 3):
+            row = base.copy()
+            ids, vals, _ = pgen_difflist(data, p, N)
+            row[ids] = vals
+            if t == 3:
+                row = inv[row]
+        else:
+            raise ValueError(f"variant {m}: .pgen record type {t} is not supported")
+        if t not in (2, 3):
+            base = row
+        out[m] = row
+    return out, N, M
+
+
 PINS = {
     "bed_to_packed / bed_to_unpacked / unpack / pack": "reader fixtures demo_data.G(.packed).expected, exact",
     "flip_packed / mean_packed": "closed form on random tiles (0<->2, padding zeroed) and demo fixture",
@@ -962,6 +1209,12 @@ PINS = {
     "snap_q / init_q_supervised / labels_to_y": "closed form",
     "adamem_gpu(labels) / brqn(labels)": "no labels -> the unsupervised path bit for bit; all labelled -> P "
                                          "satisfies the KKT conditions of max_P logL(P, Q fixed)",
+    "vcf_to_unpacked / vcf_gt_code": "the reference's reader fixture demo_data.G.expected from demo_data.vcf(.gz); "
+                                      "hand-derived codes for haploid, multi-allelic, '.', '-', short lines",
+    "bcf_to_unpacked": "demo_data.G.expected from demo_data.bcf(.gz); hand-built records (int8/int16 GT, "
+                       "missing, vector end, haploid)",
+    "pgen_to_unpacked / pgen_difflist / pgen_header": "demo_data.G.expected from demo_data.pgen (record types "
+                                                      "0, 1, 2, 3, 4 all present); structural checks of the header",
     "splitmix64 / cv_hash_folds": "splitmix64's published first outputs for seeds 0, C1, 2 C1; folds partition "
                                   "the observed entries",
     "cv_deviance_sum": "sum of deviances = -logL(held-out entries) - 2 ln 2 x #heterozygous (logL pinned above); "

@@ -4,6 +4,8 @@ Run in the development container only (the reference tree is not present on
 the GPU machine).  Inputs are data files that the reference's tests hold:
 
 * adamixture/demo/data/demo_data.{bed,bim,fam}            (N=105, M=8451)
+* adamixture/demo/data/demo_data.{vcf.gz,bcf,bcf.gz,pgen,pvar,psam} (the same genotypes;
+  the reference's tests/test_reader.py reads every format to one fixture)
 * adamixture/demo/outputs/reader/demo_data.G(.packed).expected
 * adamixture/demo/outputs/{cpu,gpu}/{adamem,brqn}/demo_run.*.expected
 
@@ -25,7 +27,7 @@ OUT = Path(__file__).resolve().parent
 def main() -> None:
     data = REF / "adamixture" / "demo" / "data"
     outs = REF / "adamixture" / "demo" / "outputs"
-    for ext in ("bed", "bim", "fam"):
+    for ext in ("bed", "bim", "fam", "vcf.gz", "bcf", "bcf.gz", "pgen", "pvar", "psam"):
         shutil.copyfile(data / f"demo_data.{ext}", OUT / f"demo_data.{ext}")
     arrays = {
         "reader_G": np.loadtxt(outs / "reader" / "demo_data.G.expected").astype(np.uint8),

@@ -81,3 +81,44 @@ def test_cv_pack_host_matches_oracle_pack():
     for M, N in ((1, 3), (5, 7), (64, 33)):
         G = np.random.default_rng(M).integers(0, 4, size=(M, N), dtype=np.uint8)
         np.testing.assert_array_equal(pack_host(G), O.pack(G))
+
+
+def test_chromosome_filter_matches_reference_counts():
+    """The reference's tests/test_reader.py expects 682 SNPs for chromosome
+    1, 1454 for 1 and 2, the same through 'chr1', all 8451 when chrom_mode is
+    'all', and none for chromosome 9999 (the CHROM column of the demo data)."""
+    from adamixture_amd.formats import ChromFilter, normalize_specific_chrom
+    from tests.conftest import GOLDEN
+    chroms = [line.split()[0].encode() for line in (GOLDEN / "demo_data.bim").read_text().splitlines()]
+    count = lambda f: sum(f(c) for c in chroms)                                  # noqa: E731
+    assert count(ChromFilter("autosomes", 22, [1])) == 682
+    assert count(ChromFilter("autosomes", 22, ["1", "2"])) == 1454
+    assert count(ChromFilter("autosomes", 22, ["chr1"])) == 682
+    assert count(ChromFilter("all", 22, [1])) == 8451
+    assert count(ChromFilter("autosomes", 22, [9999])) == 0
+    assert count(ChromFilter("autosomes", 22)) == 8451
+    assert normalize_specific_chrom("chr1, 2 X") == [1, 2, "X"]
+    f = ChromFilter("autosomes", 3)
+    assert [f(t) for t in (b"1", b"chr3", b"Chr2", b"4", b"X", b"", b"chr")] == [True, True, True, False, False,
+                                                                               False, False]
+    g = ChromFilter("autosomes", 22, ["X", "chrY"])
+    assert [g(t) for t in (b"X", b"chrX", b"Y", b"chrY", b"1")] == [True, True, True, True, False]
+    with pytest.raises(ValueError):
+        ChromFilter("some", 22)
+
+
+def test_line_blocks_and_bcf_index():
+    import gzip
+    import io
+    from adamixture_amd.formats import ChromFilter, _line_blocks, bcf_index
+    from tests.conftest import GOLDEN
+    text = b"a\tb\nccc\n\ndd"
+    for size in (1, 2, 3, 100):
+        blocks = list(_line_blocks(io.BytesIO(text), size))
+        assert all(b.endswith(b"\n") for b in blocks) and b"".join(blocks) == text + b"\n"
+    data = gzip.decompress((GOLDEN / "demo_data.bcf").read_bytes())
+    N, rs, re_, go, gd, skipped = bcf_index(data, ChromFilter("all", 22))
+    assert N == 105 and len(rs) == 8451 and skipped == 0 and np.all(gd >= 0)
+    assert np.all(go >= rs) and np.all(go + 105 * (gd & 0xFFFF) * (gd >> 16) <= re_)
+    N, rs, *_ = bcf_index(data, ChromFilter("autosomes", 22, ["1", "2"]))
+    assert len(rs) == 1454

@@ -502,6 +502,79 @@ def test_supervised_partial_labels_keep_snapped_rows(fixtures):
     assert h[-1][1] > h[0][1]
 
 
+# -------------------------------------------------------------- VCF / BCF ----
+
+def _bcf_stream(samples, records, gt_idx=None):
+    """A decompressed BCF 2.2 byte stream.  records: (contig index, GT values
+    as an int array of shape N x n, numpy dtype int8 / int16)."""
+    import struct
+    fmt = '##FORMAT=<ID=GT,Number=1,Type=String,Description="Genotype"' + \
+        (f",IDX={gt_idx}>" if gt_idx is not None else ">")
+    text = "\n".join(["##fileformat=VCFv4.3", "##contig=<ID=1>", "##contig=<ID=2>", fmt,
+                      "\t".join(["#CHROM", "POS", "ID", "REF", "ALT", "QUAL", "FILTER", "INFO", "FORMAT",
+                                 *samples])]) + "\n\0"
+    out = b"BCF\x02\x02" + struct.pack("<I", len(text)) + text.encode()
+    key = gt_idx if gt_idx is not None else 0
+    for contig, gt in records:
+        n_s, n = gt.shape
+        shared = struct.pack("<iiifII", contig, 100, 1, 0.0, 0, (1 << 24) | n_s)
+        t = {1: 1, 2: 2}[gt.dtype.itemsize]
+        indiv = bytes([0x11, key, (n << 4) | t]) + gt.astype(gt.dtype.newbyteorder("<")).tobytes()
+        out += struct.pack("<II", len(shared), len(indiv)) + shared + indiv
+    return out
+
+
+def test_vcf_bcf_oracles_match_reader_fixture(fixtures):
+    """The reference's tests/test_reader.py reads every format of the demo
+    data to one fixture (demo_data.G.expected)."""
+    import gzip
+    G = fixtures["reader_G"]
+    v, N, M = O.vcf_to_unpacked(gzip.decompress((GOLDEN / "demo_data.vcf.gz").read_bytes()))
+    assert (N, M) == (105, 8451) and np.array_equal(v, G)
+    for name in ("demo_data.bcf", "demo_data.bcf.gz"):
+        data = (GOLDEN / name).read_bytes()
+        while data[:2] == b"\x1f\x8b":
+            data = gzip.decompress(data)
+        b, N, M = O.bcf_to_unpacked(data)
+        assert np.array_equal(b, G)
+
+
+def test_vcf_gt_known_answers():
+    """Codes written out from the reference parser's rules
+    (snp_reader.pyx:305-375): '.', '-' or no digits -> missing; haploid a ->
+    2a (missing if a > 1); diploid a1 + a2, missing if > 2; fields a short
+    line lacks -> missing."""
+    head = b"##fileformat=VCFv4.3\n#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\ta\tb\tc\td\te\n"
+    body = (b"1\t1\t.\tA\tC\t.\t.\t.\tGT\t0/0\t0|1\t1/1\t./.\t0/.\n"
+            b"1\t2\t.\tA\tC\t.\t.\t.\tGT:DP\t1/2:3\t0/2:1\t.|1\t-\t\n"
+            b"2\t3\t.\tA\tC\t.\t.\t.\tGT\t0\t1\t2\t10/0\t1:9\n"
+            b"2\t4\t.\tA\tC\t.\t.\t.\tGT\t1/0\t0/0\n"
+            b"X\t5\t.\tA\tC\t.\t.\t.\tGT\t0/1\t1/1\t0/0\t0/0\t0/0")
+    want = np.array([[0, 1, 2, 3, 3], [3, 2, 3, 3, 3], [0, 2, 3, 3, 2], [1, 0, 3, 3, 3], [1, 2, 0, 0, 0]],
+                    dtype=np.uint8)
+    G, N, M = O.vcf_to_unpacked(head + body)
+    assert (N, M) == (5, 5)
+    np.testing.assert_array_equal(G, want)
+    G, _, M = O.vcf_to_unpacked(head + body, keep=lambda t: t in (b"1", b"2"))
+    np.testing.assert_array_equal(G, want[:4])
+
+
+def test_bcf_known_answers():
+    """BCF GT values are (allele + 1) << 1 | phased; 0 = missing allele,
+    -127 (int8) / -32767 (int16) = vector end; the sum of the two alleles
+    (or the single allele of a haploid vector), 3 where one is missing
+    (snp_reader.pyx:1025-1079)."""
+    enc = lambda a, ph=0: ((a + 1) << 1) | ph                                  # noqa: E731
+    r1 = np.array([[enc(0), enc(0)], [enc(0), enc(1, 1)], [enc(1), enc(1)], [0, enc(0)], [enc(1), -127]], np.int8)
+    r2 = np.array([[enc(1)], [enc(0)], [0], [enc(2)], [enc(0)]], np.int8)
+    r3 = np.array([[enc(0), enc(2)], [enc(1), enc(1)], [enc(0), 0], [enc(1), enc(0)], [-32767, -32767]], np.int16)
+    data = _bcf_stream(list("abcde"), [(0, r1), (1, r2), (0, r3)])
+    G, N, M = O.bcf_to_unpacked(data)
+    np.testing.assert_array_equal(G, [[0, 1, 2, 3, 3], [1, 0, 3, 2, 0], [2, 2, 3, 1, 3]])
+    G, _, M = O.bcf_to_unpacked(_bcf_stream(list("abcde"), [(0, r1), (1, r2)], gt_idx=5))
+    np.testing.assert_array_equal(G, [[0, 1, 2, 3, 3], [1, 0, 3, 2, 0]])
+
+
 # ------------------------------------------------------- cross-validation ----
 
 def test_splitmix64_published_outputs():
