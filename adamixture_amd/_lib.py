@@ -68,6 +68,8 @@ SIGNATURES = {
     "adx_vcf_gt": (_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p]),
     "adx_bcf_gt": (_int, [_p, _p, _p, _i64, _i64, _i64, _p, _i64, _p]),
     "adx_clear_tail_fields": (_int, [_p, _i64, _i64, _i64, _p]),
+    "adx_pgen_decode": (_int, [_p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _i64, _p]),
+    "adx_select_snps": (_int, [_p, _i64, _p, _i64, _i64, _p, _p]),
     "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p]),
     "adx_gemm_nn_f64": (_int, [_p, _i64, _i64, _int, _int, _p, _i64, _dbl, _p, _p, _dbl, _p, _i64, _p]),
     "adx_set_em_tiles": (_int, [_int, _int, _i64]),

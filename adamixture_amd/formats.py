@@ -372,12 +372,130 @@ def read_bcf_gpu(path, device="cuda", chrom_mode: str = "all", autosomes: int = 
     return _finish(G, flip), N, M
 
 
+# ------------------------------------------------------------------- PGEN ----
+
+def pgen_layout(data: bytes):
+    """Header of a variable-width (storage mode 0x10) PLINK 2 .pgen: returns
+    (M, N, record starts, record lengths, record types).  Byte 11 holds the
+    record-type width (4 or 8 bits) and the record-length width (1-4 bytes);
+    each variant block of 2^16 variants stores its types, lengths and
+    optional provisional-reference flags after the block offsets.  The
+    reference reads .pgen through pgenlib (snp_reader.pyx:1110-1135); stored
+    allele counts (multiallelic variants) are not supported here."""
+    if len(data) < 12 or data[:2] != b"\x6c\x1b":
+        raise ValueError("not a PLINK 2 .pgen file")
+    if data[2] != 0x10:
+        raise ValueError(f".pgen storage mode {data[2]:#x} is not supported (only variable-width, 0x10)")
+    M = int.from_bytes(data[3:7], "little")
+    N = int.from_bytes(data[7:11], "little")
+    ctl = data[11]
+    if ctl & 0x30:
+        raise ValueError(".pgen with stored allele counts (multiallelic variants) is not supported")
+    nblk = (M + 65535) // 65536
+    pos = 12 + 8 * nblk
+    offs = np.frombuffer(data, "<u8", nblk, 12).astype(np.int64)
+    lb = (ctl & 3) + 1
+    types, lens = [], []
+    for b in range(nblk):
+        cnt = min(65536, M - 65536 * b)
+        if (ctl >> 2) & 1:
+            t = np.frombuffer(data, np.uint8, cnt, pos)
+            pos += cnt
+        else:
+            v = np.frombuffer(data, np.uint8, (cnt + 1) // 2, pos)
+            pos += (cnt + 1) // 2
+            t = np.stack([v & 15, v >> 4], axis=1).reshape(-1)[:cnt]
+        raw = np.frombuffer(data, np.uint8, cnt * lb, pos).reshape(cnt, lb).astype(np.int64)
+        pos += cnt * lb
+        if (ctl >> 6) == 3:
+            pos += (cnt + 7) // 8
+        types.append(t)
+        lens.append((raw << (8 * np.arange(lb))).sum(axis=1))
+    types, lens = np.concatenate(types).astype(np.uint8), np.concatenate(lens)
+    starts = np.empty(M, np.int64)
+    for b in range(nblk):
+        s, e = 65536 * b, min(M, 65536 * (b + 1))
+        starts[s:e] = offs[b] + np.concatenate([[0], np.cumsum(lens[s:e])[:-1]])
+        end = offs[b] + lens[s:e].sum()
+        if end > len(data) or (b + 1 < nblk and end != offs[b + 1]):
+            raise ValueError(".pgen record lengths do not match the block offsets")
+    if M and starts[0] < pos:
+        raise ValueError(".pgen records overlap the header")
+    return M, N, starts, lens, types
+
+
+def _pgen_variant_chroms(pgen_path: Path) -> list[bytes]:
+    """CHROM column of the .pvar (or .bim) next to the .pgen, plain or .gz."""
+    stem = str(pgen_path)
+    for ext in (".pgen.gz", ".pgen.zst", ".pgen"):
+        if stem.endswith(ext):
+            stem = stem[:-len(ext)]
+            break
+    for ext in (".pvar", ".pvar.gz", ".pvar.zst", ".bim", ".bim.gz", ".bim.zst"):
+        cand = Path(stem + ext)
+        if cand.exists():
+            with open_binary(cand) as fh:
+                return [ln.split()[0] for ln in fh.read().splitlines() if ln.strip() and not ln.startswith(b"#")]
+    raise FileNotFoundError(f"Variant file (.pvar or .bim) not found for {stem}")
+
+
+def read_pgen_gpu(path, device="cuda", chrom_mode: str = "all", autosomes: int = 22, specific_chrom=None,
+                  flip: bool = True) -> tuple[PackedGenotypes, int, int]:
+    """PGEN hardcalls decoded on the GPU (adx_pgen_decode: non-LD records,
+    then the LD-compressed ones against their decoded base), the variants
+    the .pvar/.bim CHROM filter keeps gathered by adx_select_snps, then the
+    flip rule (snp_reader.py:592-682 semantics)."""
+    dev = torch.device(device)
+    keep = ChromFilter(chrom_mode, autosomes, specific_chrom)
+    with open_binary(path) as fh:
+        data = fh.read()
+    M_all, N, starts, lens, types = pgen_layout(data)
+    chroms = _pgen_variant_chroms(Path(str(path)))
+    if len(chroms) != M_all:
+        raise ValueError(f"variant file lists {len(chroms)} variants, the .pgen {M_all}")
+    kept = np.flatnonzero([keep(c) for c in chroms])
+    keep.warn(M_all - kept.size)
+    if kept.size == 0 or N == 0:
+        raise ValueError("No variants found in dataset matching chromosome filters.")
+    t = types & 7
+    if np.any((t == 5) | (t == 6) | (t == 7)):
+        raise ValueError(f".pgen record types {sorted(set(t[t >= 5].tolist()))} are not supported")
+    need = np.where(t == 0, (N + 3) // 4, np.where(t == 1, 1 + (N + 7) // 8 + 1, 1))
+    if np.any(lens < need):
+        raise ValueError(".pgen record shorter than its genotype track")
+    ld = (t == 2) | (t == 3)
+    last = np.where(~ld, np.arange(M_all), -1)
+    base = np.maximum.accumulate(last)
+    if np.any(ld & (base < 0)):
+        raise ValueError(".pgen LD-compressed record without a preceding base record")
+    idb = 1 if N <= 1 << 8 else (2 if N <= 1 << 16 else (3 if N <= 1 << 24 else 4))
+    Gall = _device_matrix(M_all, N, dev)
+    st = _lib.stream_handle(dev)
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)           # noqa: E731
+    buf = up(np.frombuffer(data, np.uint8).copy())
+    d_st, d_len, d_t, d_base = up(starts), up(lens), up(types), up(base.astype(np.int64))
+    for sel in (np.flatnonzero(~ld), np.flatnonzero(ld)):
+        if sel.size:
+            lst = up(sel.astype(np.int64))
+            _lib.call("adx_pgen_decode", buf.data_ptr(), d_st.data_ptr(), d_len.data_ptr(), d_t.data_ptr(),
+                      d_base.data_ptr(), lst.data_ptr(), sel.size, N, idb, Gall.data.data_ptr(), Gall.ldg, st)
+    if kept.size == M_all:
+        G = Gall
+    else:
+        G = _device_matrix(kept.size, N, dev)
+        idx = up(kept.astype(np.int64))
+        _lib.call("adx_select_snps", Gall.data.data_ptr(), Gall.ldg, idx.data_ptr(), kept.size, N, G.data.data_ptr(),
+                  st)
+    torch.cuda.current_stream(dev).synchronize()
+    return _finish(G, flip), N, G.M
+
+
 # ------------------------------------------------------------- dispatcher ----
 
 def read_data(tr_file, packed: bool = True, chunk_size: int = 4096, chrom_mode: str = "autosomes",
               autosomes: int = 22, specific_chrom=None, verbose: bool = True, device="cuda"):
     """utils.py:161-205 / snp_reader.py:685-770 read_data for the GPU path:
-    .bed, .vcf and .bcf (plain or .gz) go straight to the device-resident
+    .bed, .vcf, .bcf and .pgen (plain or .gz) go straight to the device-resident
     packed matrix (returned as PackedGenotypes); ``packed=False`` returns the
     unpacked M x N uint8 matrix on the host.  A filter that leaves no SNP
     exits with status 1, as the reference does.  ``chunk_size`` is accepted
@@ -401,8 +519,7 @@ def read_data(tr_file, packed: bool = True, chunk_size: int = 4096, chrom_mode: 
         elif ".bcf" in exts:
             G, N, M = read_bcf_gpu(tr_file, device, **kw)
         elif ".pgen" in exts:
-            raise RuntimeError(f"{tr_file}: PGEN input needs a PGEN decoder (the reference uses pgenlib); "
-                               "convert it to .bed, .vcf or .bcf")
+            G, N, M = read_pgen_gpu(tr_file, device, **kw)
         else:
             log.error("    Invalid format. Unrecognized file format. Make sure file ends with .bed, .pgen, "
                       ".vcf or .bcf .")

@@ -121,7 +121,7 @@ def read_bed_gpu(path: str | Path, device="cuda", chrom_mode: str = "all", autos
         keep = np.array([_keep_chrom(line.split()[0], chrom_mode, autosomes, specific_chrom)
                          for line in f if line.strip()], dtype=bool)
     nb = ceil(N / 4)
-    raw = np.memmap(f"{base}.bed", dtype=np.uint8, mode="r")
+    raw = np.memmap(f"{base}.bed", dtype=np.uint8, mode="c")       # copy-on-write: writable views, file untouched
     if raw.size < 3 or raw[0] != 0x6C or raw[1] != 0x1B or raw[2] != 0x01:
         raise ValueError(f"{base}.bed is not a SNP-major PLINK bed file")
     if (raw.size - 3) % nb or (raw.size - 3) // nb != keep.size:

@@ -333,6 +333,25 @@ int adx_vcf_gt(const uint8_t *text, const int64_t *line_start, const int64_t *li
 int adx_bcf_gt(const uint8_t *buf, const int64_t *gt_off, const int32_t *gt_desc, int64_t n_rec, int64_t m0,
                int64_t N, uint8_t *G, int64_t ldg, void *stream);
 
+/* PLINK 2 .pgen hardcalls (the reference reads them through pgenlib,
+ * snp_reader.pyx:1110-1135).  buf holds the file; record m starts at
+ * buf[rstart[m]], is rlen[m] bytes long and has type rtype[m] (low 3 bits:
+ * 0 = 2-bit array, 1 = two values + bit array + difflist, 4 = all 0 +
+ * difflist, 2 / 3 = LD-compressed against record rbase[m], 3 inverted).
+ * Decodes the records list[0 .. n) into SNP rows m of G (fields preset to 3);
+ * call once for the non-LD records, then once for the LD-compressed ones.
+ * idb = bytes per sample ID in difflists (1 for N <= 256, ... 4).  The
+ * caller checks that each record holds its dense track; difflists are
+ * bounds-checked against rlen. */
+int adx_pgen_decode(const uint8_t *buf, const int64_t *rstart, const int64_t *rlen, const uint8_t *rtype,
+                    const int64_t *rbase, const int64_t *list, int64_t n, int64_t N, int idb, uint8_t *G, int64_t ldg,
+                    void *stream);
+
+/* Gout (ceil(M_out/4) x ldg) field (m, i) = Gin field (idx[m], i); padding
+ * bytes 0xFF, tail fields 0 (the chromosome filter of snp_reader.py:657-660). */
+int adx_select_snps(const uint8_t *Gin, int64_t ldg, const int64_t *idx, int64_t M_out, int64_t N, uint8_t *Gout,
+                    void *stream);
+
 /* Fields of the last byte row beyond SNP M - 1 set to 0 (pack_genotypes,
  * snp_reader.pyx:123-157, leaves them 0). */
 int adx_clear_tail_fields(uint8_t *G, int64_t ldg, int64_t M, int64_t N, void *stream);

@@ -111,3 +111,26 @@ def test_bcf_edge_cases_match_oracle(cuda, tmp_path):
         G, N2, M2 = F.read_bcf_gpu(path, cuda, "all", 22, block_bytes=block)
         assert (N2, M2) == (N, M) == (70, 23)
         np.testing.assert_array_equal(_packed_host(G), _expect_packed(want))
+
+
+@pytest.mark.parametrize("sidecar", ["plain", "gz"])
+def test_pgen_matches_reader_fixture(cuda, fixtures, tmp_path, sidecar):
+    """Record types 0, 1, 2, 3, 4 of the demo .pgen decoded on the GPU; the
+    .pvar sidecar plain or gzipped; the chromosome filter gathers rows."""
+    (tmp_path / "d.pgen").write_bytes((GOLDEN / "demo_data.pgen").read_bytes())
+    pvar = (GOLDEN / "demo_data.pvar").read_bytes()
+    if sidecar == "gz":
+        (tmp_path / "d.pvar.gz").write_bytes(gzip.compress(pvar))
+    else:
+        (tmp_path / "d.pvar").write_bytes(pvar)
+    print(f"case: pgen sidecar {sidecar}", flush=True)
+    G, N, M = F.read_pgen_gpu(tmp_path / "d.pgen", cuda, "autosomes", 22)
+    assert (N, M) == (105, 8451)
+    np.testing.assert_array_equal(_packed_host(G), fixtures["reader_G_packed"])
+    chroms = np.array([line.split()[0] for line in (GOLDEN / "demo_data.bim").read_text().splitlines()])
+    G, N, M = F.read_data(str(tmp_path / "d.pgen"), packed=True, chrom_mode="autosomes", specific_chrom=["1", "2"],
+                          device=cuda, verbose=False)
+    assert M == 1454
+    np.testing.assert_array_equal(_packed_host(G), _expect_packed(fixtures["reader_G"][np.isin(chroms, ["1", "2"])]))
+    with pytest.raises(SystemExit):
+        F.read_data(str(tmp_path / "d.pgen"), specific_chrom=[9999], device=cuda, verbose=False)

@@ -122,3 +122,16 @@ def test_line_blocks_and_bcf_index():
     assert np.all(go >= rs) and np.all(go + 105 * (gd & 0xFFFF) * (gd >> 16) <= re_)
     N, rs, *_ = bcf_index(data, ChromFilter("autosomes", 22, ["1", "2"]))
     assert len(rs) == 1454
+
+
+def test_pgen_layout_matches_oracle_header():
+    import oracle as O
+    from adamixture_amd.formats import pgen_layout
+    from tests.conftest import GOLDEN
+    data = (GOLDEN / "demo_data.pgen").read_bytes()
+    for a, b in zip(pgen_layout(data), O.pgen_header(data)):
+        np.testing.assert_array_equal(a, b)
+    with pytest.raises(ValueError):
+        pgen_layout(data[:2] + b"\x01" + data[3:])          # fixed-width storage mode
+    with pytest.raises(ValueError):
+        pgen_layout(data[:-1])                                # truncated records

@@ -539,6 +539,18 @@ def test_vcf_bcf_oracles_match_reader_fixture(fixtures):
         assert np.array_equal(b, G)
 
 
+def test_pgen_oracle_matches_reader_fixture(fixtures):
+    """The demo .pgen holds records of types 0 (1673), 1 (272), 2 (43), 3 (1)
+    and 4 (6462); decoded they must give the reference's reader fixture."""
+    data = (GOLDEN / "demo_data.pgen").read_bytes()
+    G, N, M = O.pgen_to_unpacked(data)
+    assert (N, M) == (105, 8451)
+    np.testing.assert_array_equal(G, fixtures["reader_G"])
+    _, _, starts, lens, types = O.pgen_header(data)
+    assert starts[-1] + lens[-1] == len(data)
+    assert sorted(set((types & 7).tolist())) == [0, 1, 2, 3, 4]
+
+
 def test_vcf_gt_known_answers():
     """Codes written out from the reference parser's rules
     (snp_reader.pyx:305-375): '.', '-' or no digits -> missing; haploid a ->
