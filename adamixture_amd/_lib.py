@@ -21,6 +21,7 @@ ADX_BF16 = 2
 ADX_EM_STATS = 1
 ADX_EM_LOGLIK = 2
 ADX_CV_BLOCKS = 1024          # include/adx.h
+ADX_GEMM_TN_BLOCKS = 512      # include/adx.h
 
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -47,8 +48,9 @@ SIGNATURES = {
     "adx_em_bf16_operand_bytes": (_i64, [_i64, _i64, _int, _p]),
     "adx_em_prep_bf16": (_int, [_p, _p, _i64, _i64, _int, _p, _i64, _p]),
     "adx_em_pass_bf16": (_int, [_p, _i64, _i64, _i64, _int, _p, _i64, _p, _p, _p, _p]),
-    "adx_svd_xw": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p]),
-    "adx_svd_xty": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p]),
+    "adx_svd_xw": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p, _p]),
+    "adx_svd_xty": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p, _p]),
+    "adx_svd_work_floats": (_i64, [_int, _i64, _i64, _int]),
     "adx_grad_hess": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _p, _p, _p, _p]),
     "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
     "adx_bvls_batch": (_int, [_p, _p, _p, _p, _i64, _int, _dbl, _dbl, _p, _i64, _p]),
@@ -70,7 +72,7 @@ SIGNATURES = {
     "adx_clear_tail_fields": (_int, [_p, _i64, _i64, _i64, _p]),
     "adx_pgen_decode": (_int, [_p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _i64, _p]),
     "adx_select_snps": (_int, [_p, _i64, _p, _i64, _i64, _p, _p]),
-    "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p]),
+    "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p, _p]),
     "adx_gemm_nn_f64": (_int, [_p, _i64, _i64, _int, _int, _p, _i64, _dbl, _p, _p, _dbl, _p, _i64, _p]),
     "adx_set_em_tiles": (_int, [_int, _int, _i64]),
     "adx_get_em_tiles": (_int, [_int, _p, _p]),

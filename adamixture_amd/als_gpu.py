@@ -44,13 +44,25 @@ def _pinv_host(A: torch.Tensor) -> torch.Tensor:
     return torch.from_numpy(np.linalg.pinv(a, rcond=max(a.shape) * np.finfo(np.float64).eps)).to(A.device)
 
 
+_TN_WORK: dict = {}
+
+
+def _tn_work(dev: torch.device, a: int, b: int) -> torch.Tensor:
+    need = _lib.ADX_GEMM_TN_BLOCKS * a * b
+    w = _TN_WORK.get(dev)
+    if w is None or w.numel() < need:
+        w = _TN_WORK[dev] = torch.empty(need, dtype=torch.float64, device=dev)
+    return w
+
+
 def _tn(X: torch.Tensor, Y: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
-    """alpha X^T Y for tall X (n x a), Y (n x b) (adx_gemm_tn_f64)."""
+    """alpha X^T Y for tall X (n x a), Y (n x b) (adx_gemm_tn_f64, with a
+    workspace: fixed summation order, so ALS runs are reproducible)."""
     n, a = X.shape
     b = Y.shape[1]
     C = torch.empty((a, b), dtype=torch.float64, device=X.device)
     _lib.call("adx_gemm_tn_f64", X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0), n, a, b, alpha,
-              C.data_ptr(), b, _lib.stream_handle(X.device))
+              C.data_ptr(), b, _tn_work(X.device, a, b).data_ptr(), _lib.stream_handle(X.device))
     return C
 
 

@@ -5,8 +5,10 @@
 // large square blocks; here
 //   gemm_tn:  C = alpha X^T Y (a x b), X (n x a), Y (n x b): each workgroup
 //             reduces a range of rows with the a x b partial sums spread over
-//             its threads (row chunks staged in LDS), then one FP64 atomic
-//             per output and workgroup;
+//             its threads (row chunks staged in LDS) and stores them in its
+//             slice of the caller's workspace; one more kernel sums the
+//             slices in a fixed order (deterministic; without a workspace,
+//             one FP64 atomic per output and workgroup);
 //   gemm_nn:  C = alpha X W + u v^T (+ beta C), X (n x a), W (a x b) small:
 //             W (and v) staged in LDS, one thread per (row, output column
 //             group), the row of X read once per thread group.
@@ -30,7 +32,8 @@ __global__ void zero_block_kernel(double* __restrict__ C, int a, int b, int64_t 
 __global__ __launch_bounds__(256) void gemm_tn_kernel(const double* __restrict__ X, int64_t ldx,
                                                        const double* __restrict__ Y, int64_t ldy, int64_t n, int a,
                                                        int b, double alpha, int64_t rows_per_block,
-                                                       double* __restrict__ C, int64_t ldc) {
+                                                       double* __restrict__ C, int64_t ldc,
+                                                       double* __restrict__ part) {
     __shared__ double sx[TN_ROWS][64];
     __shared__ double sy[TN_ROWS][64];
     const int tid = threadIdx.x;
@@ -62,10 +65,25 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const double* __restrict__
     for (int q = 0; q < 4; ++q) {
         const int o = tid + 256 * q;
         if (o < nout) {
-            const int i = o / b, j = o - i * b;
-            atomicAdd(&C[i * ldc + j], alpha * acc[q]);
+            if (part) {
+                part[(int64_t)blockIdx.x * nout + o] = alpha * acc[q];
+            } else {
+                const int i = o / b, j = o - i * b;
+                atomicAdd(&C[i * ldc + j], alpha * acc[q]);
+            }
         }
     }
+}
+
+// C[i][j] = sum over the nblk slices of part[blk * a b + i b + j], in order.
+__global__ void gemm_tn_reduce_kernel(const double* __restrict__ part, int64_t nblk, int a, int b,
+                                      double* __restrict__ C, int64_t ldc) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nout = a * b;
+    if (o >= nout) return;
+    double s = 0.0;
+    for (int64_t k = 0; k < nblk; ++k) s += part[k * nout + o];
+    C[(int64_t)(o / b) * ldc + o % b] = s;
 }
 
 // C[r][j] = alpha sum_i X[r][i] W[i][j] + u[r] v[j] + beta C[r][j]
@@ -109,18 +127,26 @@ using namespace adx;
 extern "C" {
 
 int adx_gemm_tn_f64(const double* X, int64_t ldx, const double* Y, int64_t ldy, int64_t n, int a, int b,
-                    double alpha, double* C, int64_t ldc, void* stream) {
+                    double alpha, double* C, int64_t ldc, double* work, void* stream) {
     if (a < 1 || b < 1 || a > 64 || b > 64 || a * b > 1024)
         return fail_msg(1, "adx_gemm_tn_f64: need 1 <= a, b <= 64 and a * b <= 1024");
     if (ldx < a || ldy < b || ldc < b) return fail_msg(1, "adx_gemm_tn_f64: bad stride");
     hipStream_t st = (hipStream_t)stream;
-    // C is overwritten: zero it with a kernel (graph-safe), then accumulate.
-    zero_block_kernel<<<(unsigned)ceil_div(a * b, 256), 256, 0, st>>>(C, a, b, ldc);
+    int64_t nblk = 0, rpb = TN_ROWS;
     if (n > 0) {
         int64_t blocks = ceil_div(n, 4 * TN_ROWS);
-        if (blocks > 512) blocks = 512;
-        const int64_t rpb = ceil_div(ceil_div(n, blocks), TN_ROWS) * TN_ROWS;
-        gemm_tn_kernel<<<(unsigned)ceil_div(n, rpb), 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc);
+        if (blocks > ADX_GEMM_TN_BLOCKS) blocks = ADX_GEMM_TN_BLOCKS;
+        rpb = ceil_div(ceil_div(n, blocks), TN_ROWS) * TN_ROWS;
+        nblk = ceil_div(n, rpb);
+    }
+    if (work && nblk > 0) {
+        gemm_tn_kernel<<<(unsigned)nblk, 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc, work);
+        gemm_tn_reduce_kernel<<<(unsigned)ceil_div(a * b, 256), 256, 0, st>>>(work, nblk, a, b, C, ldc);
+    } else {
+        // C is overwritten: zero it with a kernel (graph-safe), then accumulate.
+        zero_block_kernel<<<(unsigned)ceil_div(a * b, 256), 256, 0, st>>>(C, a, b, ldc);
+        if (nblk > 0)
+            gemm_tn_kernel<<<(unsigned)nblk, 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc, nullptr);
     }
     ADX_LAUNCH_CHECK("adx_gemm_tn_f64");
     return 0;

@@ -7,6 +7,10 @@
 //
 //   svd_xw_kernel:   Y = X W          (M x L; W is N x L)    reduction over samples
 //   svd_xty_kernel:  A += X^T Y       (N x L; Y is M x L)    reduction over SNPs
+// When the grid splits the reduction, each split stores its slice of the
+// result in the caller's workspace and svd_sum_kernel adds the slices in a
+// fixed order (deterministic); without a workspace the splits meet in
+// FP32 atomics.
 //
 // Both run 16x16x4 FP32 MFMA tiles with L padded to 16*LT rows:
 //   xw:  Y^T (L x SNPs) += W^T (L x 4 samples) . X^T (4 samples x SNPs).  The
@@ -31,7 +35,7 @@ template <int LT, int CW>
 __global__ __launch_bounds__(256) void svd_xw_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
                                                      int64_t N, const float* __restrict__ f,
                                                      const float* __restrict__ W, int L, int64_t n_per_split,
-                                                     float* __restrict__ Y) {
+                                                     float* __restrict__ Y, float* __restrict__ part) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
@@ -101,7 +105,12 @@ __global__ __launch_bounds__(256) void svd_xw_kernel(const uint8_t* __restrict__
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int li = 16 * lt + 4 * g + j;
-                if (li < L) atomicAdd(&Y[snp * L + li], acc[cb][lt][j]);
+                if (li < L) {
+                    if (part)   // this sample split's slice, summed in order by svd_sum_kernel
+                        part[((int64_t)blockIdx.y * M + snp) * L + li] = acc[cb][lt][j];
+                    else
+                        atomicAdd(&Y[snp * L + li], acc[cb][lt][j]);
+                }
             }
     }
 }
@@ -112,7 +121,7 @@ template <int LT>
 __global__ __launch_bounds__(256) void svd_xty_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
                                                       int64_t N, const float* __restrict__ f,
                                                       const float* __restrict__ Y, int L, int64_t rows_per_split,
-                                                      float* __restrict__ A) {
+                                                      float* __restrict__ A, float* __restrict__ part) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
@@ -159,14 +168,32 @@ __global__ __launch_bounds__(256) void svd_xty_kernel(const uint8_t* __restrict_
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int li = 16 * lt + 4 * g + j;
-                if (li < L) atomicAdd(&A[smp * L + li], acc[lt][j]);
+                if (li < L) {
+                    if (part)   // this SNP split's slice, summed in order by svd_sum_kernel
+                        part[((int64_t)blockIdx.y * N + smp) * L + li] = acc[lt][j];
+                    else
+                        atomicAdd(&A[smp * L + li], acc[lt][j]);
+                }
             }
     }
 }
 
-template <int LT>
-static void launch_xw(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f,
-                      const float* W, int L, float* Y) {
+// out[e] = sum over the nparts slices of part[p * n + e], in order.
+__global__ void svd_sum_kernel(const float* __restrict__ part, int64_t nparts, int64_t n, float* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.0f;
+    for (int64_t p = 0; p < nparts; ++p) s += part[p * n + e];
+    out[e] = s;
+}
+
+// Launch shapes: xw splits the samples, xty the byte rows, until the grid
+// holds about 2048 workgroups.
+struct SvdGrid {
+    int64_t nbx, splits, per_split;
+};
+
+static SvdGrid xw_grid(int64_t M, int64_t N) {
     constexpr int CW = 4;
     const int64_t nbx = ceil_div(M, 64 * CW);
     int64_t s = ceil_div(2048, nbx);
@@ -174,13 +201,10 @@ static void launch_xw(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, 
     s = s > max_s ? max_s : (s < 1 ? 1 : s);
     if (s > 65535) s = 65535;
     const int64_t nps = ceil_div(ceil_div(N, s), 64) * 64;
-    dim3 grid((unsigned)nbx, (unsigned)ceil_div(N, nps));
-    svd_xw_kernel<LT, CW><<<grid, 256, 0, st>>>(G, ldg, M, N, f, W, L, nps, Y);
+    return {nbx, ceil_div(N, nps), nps};
 }
 
-template <int LT>
-static void launch_xty(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f,
-                       const float* Y, int L, float* A) {
+static SvdGrid xty_grid(int64_t M, int64_t N) {
     const int64_t Mb = ceil_div(M, 4);
     const int64_t nbx = ceil_div(N, 64);
     int64_t s = ceil_div(2048, nbx);
@@ -188,8 +212,30 @@ static void launch_xty(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M,
     s = s > max_s ? max_s : (s < 1 ? 1 : s);
     if (s > 65535) s = 65535;
     const int64_t rps = ceil_div(ceil_div(Mb, s), 64) * 64;
-    dim3 grid((unsigned)nbx, (unsigned)ceil_div(Mb, rps));
-    svd_xty_kernel<LT><<<grid, 256, 0, st>>>(G, ldg, M, N, f, Y, L, rps, A);
+    return {nbx, ceil_div(Mb, rps), rps};
+}
+
+template <int LT>
+static void launch_xw(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f,
+                      const float* W, int L, float* Y, float* work) {
+    constexpr int CW = 4;
+    const SvdGrid gd = xw_grid(M, N);
+    float* part = (work && gd.splits > 1) ? work : nullptr;
+    dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
+    svd_xw_kernel<LT, CW><<<grid, 256, 0, st>>>(G, ldg, M, N, f, W, L, gd.per_split, Y, part);
+    if (part)
+        svd_sum_kernel<<<(unsigned)ceil_div(M * L, 256), 256, 0, st>>>(part, gd.splits, M * (int64_t)L, Y);
+}
+
+template <int LT>
+static void launch_xty(hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f,
+                       const float* Y, int L, float* A, float* work) {
+    const SvdGrid gd = xty_grid(M, N);
+    float* part = (work && gd.splits > 1) ? work : nullptr;
+    dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
+    svd_xty_kernel<LT><<<grid, 256, 0, st>>>(G, ldg, M, N, f, Y, L, gd.per_split, A, part);
+    if (part)
+        svd_sum_kernel<<<(unsigned)ceil_div(N * L, 256), 256, 0, st>>>(part, gd.splits, N * (int64_t)L, A);
 }
 
 }  // namespace adx
@@ -198,33 +244,39 @@ using namespace adx;
 
 extern "C" {
 
+int64_t adx_svd_work_floats(int which, int64_t M, int64_t N, int L) {
+    if (M <= 0 || N <= 0 || L < 1) return 0;
+    const SvdGrid gd = which == 0 ? xw_grid(M, N) : xty_grid(M, N);
+    return gd.splits > 1 ? gd.splits * (which == 0 ? M : N) * (int64_t)L : 0;
+}
+
 int adx_svd_xw(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f, const float* W, int L,
-               float* Y, void* stream) {
+               float* Y, float* work, void* stream) {
     if (M <= 0 || N <= 0) return 0;
     if (L < 1 || L > 64) return fail_msg(1, "adx_svd_xw: L must be in [1, 64]");
     if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64) return fail_msg(1, "adx_svd_xw: bad ldg");
     hipStream_t st = (hipStream_t)stream;
     switch ((L + 15) / 16) {
-        case 1: launch_xw<1>(st, G, ldg, M, N, f, W, L, Y); break;
-        case 2: launch_xw<2>(st, G, ldg, M, N, f, W, L, Y); break;
-        case 3: launch_xw<3>(st, G, ldg, M, N, f, W, L, Y); break;
-        default: launch_xw<4>(st, G, ldg, M, N, f, W, L, Y); break;
+        case 1: launch_xw<1>(st, G, ldg, M, N, f, W, L, Y, work); break;
+        case 2: launch_xw<2>(st, G, ldg, M, N, f, W, L, Y, work); break;
+        case 3: launch_xw<3>(st, G, ldg, M, N, f, W, L, Y, work); break;
+        default: launch_xw<4>(st, G, ldg, M, N, f, W, L, Y, work); break;
     }
     ADX_LAUNCH_CHECK("adx_svd_xw");
     return 0;
 }
 
 int adx_svd_xty(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, const float* f, const float* Y, int L,
-                float* A, void* stream) {
+                float* A, float* work, void* stream) {
     if (M <= 0 || N <= 0) return 0;
     if (L < 1 || L > 64) return fail_msg(1, "adx_svd_xty: L must be in [1, 64]");
     if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64) return fail_msg(1, "adx_svd_xty: bad ldg");
     hipStream_t st = (hipStream_t)stream;
     switch ((L + 15) / 16) {
-        case 1: launch_xty<1>(st, G, ldg, M, N, f, Y, L, A); break;
-        case 2: launch_xty<2>(st, G, ldg, M, N, f, Y, L, A); break;
-        case 3: launch_xty<3>(st, G, ldg, M, N, f, Y, L, A); break;
-        default: launch_xty<4>(st, G, ldg, M, N, f, Y, L, A); break;
+        case 1: launch_xty<1>(st, G, ldg, M, N, f, Y, L, A, work); break;
+        case 2: launch_xty<2>(st, G, ldg, M, N, f, Y, L, A, work); break;
+        case 3: launch_xty<3>(st, G, ldg, M, N, f, Y, L, A, work); break;
+        default: launch_xty<4>(st, G, ldg, M, N, f, Y, L, A, work); break;
     }
     ADX_LAUNCH_CHECK("adx_svd_xty");
     return 0;

@@ -33,12 +33,27 @@ def eigSVD_host(C: np.ndarray):
     return (np.ascontiguousarray(U[:, ::-1]), np.ascontiguousarray(S[::-1]), np.ascontiguousarray(V[:, ::-1]))
 
 
+_WORK: dict = {}
+
+
+def _work(G: PackedGenotypes, which: int, L: int):
+    """Workspace for the fixed-order split reduction (None if the grid does
+    not split the reduction)."""
+    n = _lib.load().adx_svd_work_floats(which, G.M, G.N, L)
+    if n <= 0:
+        return None
+    w = _WORK.get(G.device)
+    if w is None or w.numel() < n:
+        w = _WORK[G.device] = torch.empty(n, dtype=torch.float32, device=G.device)
+    return w
+
+
 def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
     """Y = X W (M x L) with X the centred genotypes, W (N x L) FP32."""
     L = W.shape[1]
     Y = torch.zeros((G.M, L), dtype=torch.float32, device=G.device)
     _lib.call("adx_svd_xw", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.contiguous().data_ptr(), L,
-              Y.data_ptr(), _lib.stream_handle(G.device))
+              Y.data_ptr(), _lib.ptr(_work(G, 0, L)), _lib.stream_handle(G.device))
     return Y
 
 
@@ -47,7 +62,7 @@ def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
     L = Y.shape[1]
     A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
     _lib.call("adx_svd_xty", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.contiguous().data_ptr(), L,
-              A.data_ptr(), _lib.stream_handle(G.device))
+              A.data_ptr(), _lib.ptr(_work(G, 1, L)), _lib.stream_handle(G.device))
     return A
 
 

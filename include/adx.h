@@ -177,7 +177,12 @@ int adx_adam_step_sched(double *P, double *Q, const double *P_EM, const double *
  * alpha X W + u v^T + beta C with X (n x a), W (a x b), a <= 64, b <= 16,
  * u (n) and v (b) both NULL or both given.  Row-major with row strides. */
 int adx_gemm_tn_f64(const double *X, int64_t ldx, const double *Y, int64_t ldy, int64_t n, int a, int b,
-                    double alpha, double *C, int64_t ldc, void *stream);
+                    double alpha, double *C, int64_t ldc, double *work, void *stream);
+
+/* Workspace of adx_gemm_tn_f64: ADX_GEMM_TN_BLOCKS * a * b doubles give a
+ * deterministic result (per-workgroup partial sums reduced in a fixed
+ * order); work = NULL reduces with FP64 atomics (order varies run to run). */
+#define ADX_GEMM_TN_BLOCKS 512
 int adx_gemm_nn_f64(const double *X, int64_t ldx, int64_t n, int a, int b, const double *W, int64_t ldw,
                     double alpha, const double *u, const double *v, double beta, double *C, int64_t ldc,
                     void *stream);
@@ -187,13 +192,17 @@ int adx_gemm_nn_f64(const double *X, int64_t ldx, int64_t n, int a, int b, const
 /* X is the centred genotype matrix of pack2bit.cu:46-77 (x = g - 2 f[m], 0 if
  * missing), decoded inside the GEMM.  Replace the chunked unpack + SGEMM of
  * adamixture/model/svd_gpu.py:104-107 and :133-138:
- *   adx_svd_xw:  Y (M x L, row-major) += X W, W is N x L
- *   adx_svd_xty: A (N x L, row-major) += X^T Y, Y is M x L
- * Outputs are accumulated (atomically): zero them first.  1 <= L <= 64. */
+ *   adx_svd_xw:  Y (M x L, row-major) = X W, W is N x L
+ *   adx_svd_xty: A (N x L, row-major) = X^T Y, Y is M x L
+ * Zero the output first.  1 <= L <= 64.  With a workspace of
+ * adx_svd_work_floats(0 for xw / 1 for xty, M, N, L) floats the split
+ * reduction is summed in a fixed order (deterministic); work = NULL adds
+ * the splits with FP32 atomics. */
 int adx_svd_xw(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float *f, const float *W, int L,
-               float *Y, void *stream);
+               float *Y, float *work, void *stream);
 int adx_svd_xty(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float *f, const float *Y, int L,
-                float *A, void *stream);
+                float *A, float *work, void *stream);
+int64_t adx_svd_work_floats(int which, int64_t M, int64_t N, int L);
 
 /* --- SQP block-relaxation gradient / Hessian (FP64, 1 <= K <= 16) ---------- */
 

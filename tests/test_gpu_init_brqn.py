@@ -50,6 +50,30 @@ def test_svd_gemms_match_dense(cuda):
     np.testing.assert_allclose(got_a, X.T @ Y, rtol=1e-4, atol=1e-3)
 
 
+def test_init_products_are_deterministic(cuda):
+    """The split reductions of the SVD GEMMs (several splits at this shape)
+    and the ALS X^T Y products are summed in a fixed order: repeated calls
+    agree bit for bit."""
+    from adamixture_amd import _lib
+    from adamixture_amd.als_gpu import _tn
+    rng = np.random.default_rng(1)
+    M, N, L = 20000, 3000, 20
+    P = rng.uniform(0.05, 0.95, size=(M, 4))
+    Q = rng.dirichlet(np.ones(4), size=N)
+    Gp = O.synth_packed(P, Q, 2, 0.02)
+    G = as_packed(torch.from_numpy(Gp), M, cuda)
+    assert _lib.load().adx_svd_work_floats(0, M, N, L) > 0 and _lib.load().adx_svd_work_floats(1, M, N, L) > 0
+    fd = torch.from_numpy(O.freq_gpu(Gp, M)).to(cuda)
+    W = torch.from_numpy(rng.normal(size=(N, L)).astype(np.float32)).to(cuda)
+    Y = torch.from_numpy(rng.normal(size=(M, L)).astype(np.float32)).to(cuda)
+    assert torch.equal(xw(G, fd, W), xw(G, fd, W)) and torch.equal(xty(G, fd, Y), xty(G, fd, Y))
+    Xd = torch.from_numpy(rng.normal(size=(300000, 10))).to(cuda)
+    Yd = torch.from_numpy(rng.normal(size=(300000, 12))).to(cuda)
+    C1, C2 = _tn(Xd, Yd), _tn(Xd, Yd)
+    assert torch.equal(C1, C2)
+    np.testing.assert_allclose(C1.cpu().numpy(), Xd.cpu().numpy().T @ Yd.cpu().numpy(), rtol=1e-10, atol=1e-9)
+
+
 def test_svd_matches_cpu_reference_fixture(demo, fixtures):
     G, N, M = demo
     f = calculate_frequencies_gpu(G, M, 8192, G.device)
