@@ -106,7 +106,14 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
 
-    __shared__ T s_tile[4][256];
+    // Per-wave 16 x 16 transpose tile for the T contraction, rows padded to
+    // TSX = 17 elements: the column writes (lane (g, c) -> row c) and the row
+    // reads (lane (g, c) -> column c) then fall on distinct banks within each
+    // 16-lane (FP64) or 32-lane (FP32, at most 2-way) LDS pass, where an
+    // unpadded 16-element stride puts the 16 lanes of a column write on one
+    // bank.
+    constexpr int TSX = 17;
+    __shared__ T s_tile[4][16 * TSX];
     // Per wave: T of the 64 samples of the current row block over the wave's
     // SNPs.  Each wave writes each (sample, k) once per row block, so the
     // slices need no atomics and are summed in a fixed order at the flush.
@@ -207,12 +214,12 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                     for (int j = 0; j < 4; ++j) {
                         accA[cb] = MM::run(qT[j], a[j], accA[cb]);
                         accB[cb] = MM::run(qT[j], b[j], accB[cb]);
-                        s_tile[wave][cl * 16 + MM::row(g, j)] = a[j] - b[j];
+                        s_tile[wave][cl * TSX + MM::row(g, j)] = a[j] - b[j];
                         sb[j] += b[j];
                     }
 #pragma unroll
                     for (int kk = 0; kk < 4; ++kk)
-                        accT = MM::run(s_tile[wave][(4 * kk + g) * 16 + cl], pT[cb][kk], accT);
+                        accT = MM::run(s_tile[wave][(4 * kk + g) * TSX + cl], pT[cb][kk], accT);
                 }
             }
 
