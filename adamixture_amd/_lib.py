@@ -53,6 +53,7 @@ SIGNATURES = {
     "adx_svd_work_floats": (_i64, [_int, _i64, _i64, _int]),
     "adx_grad_hess": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _p, _p, _p, _p]),
     "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
+    "adx_bvls_workspace_bytes": (_i64, [_int]),
     "adx_bvls_batch": (_int, [_p, _p, _p, _p, _i64, _int, _dbl, _dbl, _p, _i64, _p]),
     "adx_probe_mfma": (_int, [_int, _p, _p, _p, _p]),
     "adx_adam_step_sched": (_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _dbl, _dbl, _dbl, _p, _p,

@@ -75,15 +75,18 @@ __global__ __launch_bounds__(256) void gemm_tn_kernel(const double* __restrict__
     }
 }
 
-// C[i][j] = sum over the nblk slices of part[blk * a b + i b + j], in order.
-__global__ void gemm_tn_reduce_kernel(const double* __restrict__ part, int64_t nblk, int a, int b,
-                                      double* __restrict__ C, int64_t ldc) {
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+// C[i][j] = sum over the nblk slices of part[blk * a b + i b + j]: one
+// wavefront per output, lane l adds slices l, l + 64, ... in order, then a
+// fixed shuffle tree (deterministic).
+__global__ __launch_bounds__(64) void gemm_tn_reduce_kernel(const double* __restrict__ part, int64_t nblk, int a,
+                                                            int b, double* __restrict__ C, int64_t ldc) {
+    const int o = blockIdx.x;
     const int nout = a * b;
-    if (o >= nout) return;
     double s = 0.0;
-    for (int64_t k = 0; k < nblk; ++k) s += part[k * nout + o];
-    C[(int64_t)(o / b) * ldc + o % b] = s;
+    for (int64_t k = threadIdx.x; k < nblk; k += 64) s += part[k * nout + o];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (threadIdx.x == 0) C[(int64_t)(o / b) * ldc + o % b] = s;
 }
 
 // C[r][j] = alpha sum_i X[r][i] W[i][j] + u[r] v[j] + beta C[r][j]
@@ -141,7 +144,7 @@ int adx_gemm_tn_f64(const double* X, int64_t ldx, const double* Y, int64_t ldy, 
     }
     if (work && nblk > 0) {
         gemm_tn_kernel<<<(unsigned)nblk, 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc, work);
-        gemm_tn_reduce_kernel<<<(unsigned)ceil_div(a * b, 256), 256, 0, st>>>(work, nblk, a, b, C, ldc);
+        gemm_tn_reduce_kernel<<<(unsigned)(a * b), 64, 0, st>>>(work, nblk, a, b, C, ldc);
     } else {
         // C is overwritten: zero it with a kernel (graph-safe), then accumulate.
         zero_block_kernel<<<(unsigned)ceil_div(a * b, 256), 256, 0, st>>>(C, a, b, ldc);

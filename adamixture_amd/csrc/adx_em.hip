@@ -117,7 +117,9 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
     // Per wave: T of the 64 samples of the current row block over the wave's
     // SNPs.  Each wave writes each (sample, k) once per row block, so the
     // slices need no atomics and are summed in a fixed order at the flush.
-    __shared__ T s_tsum[STATS ? 4 : 1][64 * 16];
+    // Rows padded to 17 elements: the 64 lanes of a write (4 lane groups x
+    // 16 columns, rows 16 apart) fall on distinct banks.
+    __shared__ T s_tsum[STATS ? 4 : 1][64 * 17];
     __shared__ double s_ll[4];
 
     // P operands, constant for the whole strip.
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                     v += __shfl_xor(v, 2);
                     v += __shfl_xor(v, 4);
                     v += __shfl_xor(v, 8);
-                    if (cl < K) s_tsum[wave][(16 * g + 4 * rw + j) * 16 + cl] = accT[j] + v;
+                    if (cl < K) s_tsum[wave][(16 * g + 4 * rw + j) * 17 + cl] = accT[j] + v;
                 }
             }
         }
@@ -247,8 +249,9 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
             for (int e = threadIdx.x; e < 64 * 16; e += 256) {
                 const int kp = e & 15;
                 const int64_t smp = s0 + (e >> 4);
+                const int ex = (e >> 4) * 17 + kp;
                 if (kp < K && smp < N) {
-                    const T v = (s_tsum[0][e] + s_tsum[1][e]) + (s_tsum[2][e] + s_tsum[3][e]);
+                    const T v = (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
                     if (Tpart)
                         Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = v;
                     else

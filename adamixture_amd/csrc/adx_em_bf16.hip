@@ -90,7 +90,8 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     // Per wave: T of the 64 samples of the current row block over the wave's
     // SNPs; each (sample, k) is written once per wave and row block, and the
     // four slices are summed in a fixed order at the flush (deterministic).
-    __shared__ float s_tsum[4][64 * 16];
+    // Rows padded to 17 floats so the 64 lanes of a write hit distinct banks.
+    __shared__ float s_tsum[4][64 * 17];
 
     const bf16x8 zero8 = bf16x8{};
     bf16x8 pH[CW][KH];
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (cl < K) s_tsum[wave][(16 * g + 4 * (2 * tp + rr) + j) * 16 + cl] = accT[rr][j];
+                    if (cl < K) s_tsum[wave][(16 * g + 4 * (2 * tp + rr) + j) * 17 + cl] = accT[rr][j];
         }
 
         __syncthreads();
@@ -230,8 +231,9 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         for (int e = threadIdx.x; e < 64 * 16; e += 256) {
             const int kp = e & 15;
             const int64_t smp = s0 + (e >> 4);
+            const int ex = (e >> 4) * 17 + kp;
             if (kp < K && smp < N) {
-                const float v = (s_tsum[0][e] + s_tsum[1][e]) + (s_tsum[2][e] + s_tsum[3][e]);
+                const float v = (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
                 if (Tpart)
                     Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = v;
                 else

@@ -392,6 +392,180 @@ __global__ __launch_bounds__(64) void bvls_kernel(const double* __restrict__ A, 
     }
 }
 
+// ---- shared-A BVLS for small K: one thread per problem ----------------
+//
+// Every problem of a batch shares A, so the inverse of each principal
+// submatrix A_FF (F a free set, 2^K of them) is computed once per batch by
+// bvls_subset_inv_kernel, with the pivoting of solve_subset_par (largest
+// |pivot| of the column, first on ties; a pivot below 1e-12 marks F
+// singular).  bvls_thread_kernel then runs the block-principal-pivoting
+// loop of bvls_kernel per thread: the solve for the free variables becomes
+// x_F = inv(A_FF) bs_F, the rest (start x0 = inv(A) b, the bound and
+// gradient tests, the iteration limit, the break on a singular free set) is
+// unchanged.  x_F then differs from the Gauss-Jordan result only by
+// rounding.
+
+constexpr int BVLS_FAST_K = 12;
+
+// One wavefront per free set S (bit i = variable i free); table slot S holds
+// inv(A_SS) (n x n, rows and columns in ascending variable order) in a
+// K x K block; sing[S] = 1 if the elimination hit a pivot below 1e-12.
+__global__ __launch_bounds__(64) void bvls_subset_inv_kernel(const double* __restrict__ A, int K,
+                                                             double* __restrict__ table, int* __restrict__ sing) {
+    __shared__ double aug[BVLS_FAST_K * 2 * BVLS_FAST_K];
+    __shared__ double mult[BVLS_FAST_K];
+    __shared__ int mi[BVLS_FAST_K];
+    __shared__ int scal[2];
+    const int lane = threadIdx.x;
+    const unsigned S = blockIdx.x;
+    if (lane == 0) {
+        int n = 0;
+        for (int i = 0; i < K; ++i)
+            if ((S >> i) & 1u) mi[n++] = i;
+        scal[0] = n;
+    }
+    __syncthreads();
+    const int n = scal[0], w = 2 * n;
+    for (int e = lane; e < n * w; e += 64) {
+        const int i = e / w, j = e - i * w;
+        aug[e] = j < n ? A[mi[i] * K + mi[j]] : (j - n == i ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int i = 0; i < n; ++i) {
+        if (lane == 0) {
+            int mr = i;
+            double mv = fabs(aug[i * w + i]);
+            for (int k = i + 1; k < n; ++k) {
+                const double t = fabs(aug[k * w + i]);
+                if (t > mv) {
+                    mv = t;
+                    mr = k;
+                }
+            }
+            scal[1] = mv < 1e-12 ? -1 : mr;
+        }
+        __syncthreads();
+        const int mr = scal[1];
+        if (mr < 0) {
+            if (lane == 0) sing[S] = 1;
+            return;
+        }
+        if (mr != i)
+            for (int j = i + lane; j < w; j += 64) {
+                const double t = aug[i * w + j];
+                aug[i * w + j] = aug[mr * w + j];
+                aug[mr * w + j] = t;
+            }
+        __syncthreads();
+        const double piv = aug[i * w + i];
+        __syncthreads();
+        for (int j = i + lane; j < w; j += 64) aug[i * w + j] /= piv;
+        for (int k = lane; k < n; k += 64) mult[k] = aug[k * w + i];
+        __syncthreads();
+        for (int e = lane; e < n * (w - i); e += 64) {
+            const int k = e / (w - i), j = i + (e - k * (w - i));
+            if (k != i) aug[k * w + j] -= mult[k] * aug[i * w + j];
+        }
+        __syncthreads();
+    }
+    for (int e = lane; e < n * n; e += 64) {
+        const int r = e / n, c = e - r * n;
+        table[(int64_t)S * K * K + r * n + c] = aug[r * w + n + c];
+    }
+    if (lane == 0) sing[S] = 0;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void bvls_thread_kernel(const double* __restrict__ A, const double* __restrict__ Ainv,
+                                                          const double* __restrict__ B, double* __restrict__ X,
+                                                          int64_t nprob, int K, double lo, double hi,
+                                                          const double* __restrict__ table,
+                                                          const int* __restrict__ sing) {
+    __shared__ double sA[KM * KM], sAi[KM * KM];
+    for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
+        sA[e] = A[e];
+        sAi[e] = Ainv[e];
+    }
+    __syncthreads();
+    const int64_t pid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= nprob) return;
+    double b[KM], x[KM], bs[KM], y[KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i) b[i] = i < K ? B[pid * K + i] : 0.0;
+#pragma unroll
+    for (int i = 0; i < KM; ++i) {
+        double s = 0.0;
+        if (i < K)
+            for (int j = 0; j < K; ++j) s += b[j] * sAi[i * K + j];
+        x[i] = s;
+    }
+    unsigned F = (1u << K) - 1u, U = 0u;
+    bool changed = false;
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+        if (i < K) {
+            if (x[i] < lo - 1e-8) {
+                F &= ~(1u << i); changed = true;
+            } else if (x[i] > hi + 1e-8) {
+                F &= ~(1u << i); U |= 1u << i; changed = true;
+            }
+        }
+    if (changed) {
+        for (int it = 1; it < 50; ++it) {
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+                if (i < K && ((F >> i) & 1u)) {
+                    double s = b[i];
+                    for (int j = 0; j < K; ++j)
+                        if (!((F >> j) & 1u)) s -= sA[i * K + j] * (((U >> j) & 1u) ? hi : lo);
+                    bs[i] = s;
+                }
+            if (sing[F]) break;
+            const int n = __popc(F);
+            const double* inv = table + (int64_t)F * K * K;
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+                if (i < K && ((F >> i) & 1u)) {
+                    const int r = __popc(F & ((1u << i) - 1u));
+                    double s = 0.0;
+                    int c = 0;
+#pragma unroll
+                    for (int j = 0; j < KM; ++j)
+                        if (j < K && ((F >> j) & 1u)) s += inv[r * n + c++] * bs[j];
+                    x[i] = s;
+                }
+#pragma unroll
+            for (int i = 0; i < KM; ++i)
+                if (i < K && !((F >> i) & 1u)) x[i] = ((U >> i) & 1u) ? hi : lo;
+#pragma unroll
+            for (int i = 0; i < KM; ++i) {
+                double s = -b[i];
+                if (i < K)
+                    for (int j = 0; j < K; ++j) s += sA[i * K + j] * x[j];
+                y[i] = s;
+            }
+            changed = false;
+#pragma unroll
+            for (int i = 0; i < KM; ++i) {
+                if (i >= K) continue;
+                const unsigned bit = 1u << i;
+                if (F & bit) {
+                    if (x[i] < lo - 1e-8) { F &= ~bit; U &= ~bit; changed = true; }
+                    else if (x[i] > hi + 1e-8) { F &= ~bit; U |= bit; changed = true; }
+                } else if (!(U & bit)) {
+                    if (y[i] < -1e-8) { F |= bit; U &= ~bit; changed = true; }
+                } else if (y[i] > 1e-8) {
+                    F |= bit; U &= ~bit; changed = true;
+                }
+            }
+            if (!changed) break;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < KM; ++i)
+        if (i < K) X[pid * K + i] = x[i];
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ULL; }
 
 static int persistent_blocks(int64_t nprob) {
@@ -423,12 +597,33 @@ int adx_sqp_solve(int simplex, const double* H, const double* r, const double* X
     return 0;
 }
 
+int64_t adx_bvls_workspace_bytes(int K) {
+    if (K < 1 || K > BVLS_FAST_K) return 8;
+    const int64_t sets = (int64_t)1 << K;
+    return 256 + sets * K * K * (int64_t)sizeof(double) + sets * (int64_t)sizeof(int);
+}
+
 int adx_bvls_batch(const double* A, const double* Ainv, const double* B, double* X, int64_t nprob, int K,
                    double lo, double hi, void* workspace, int64_t workspace_bytes, void* stream) {
     if (K < 1 || K > 64) return fail_msg(1, "adx_bvls_batch: K must be in [1, 64]");
     if (nprob <= 0) return 0;
     if (!workspace || workspace_bytes < 8) return fail_msg(1, "adx_bvls_batch: needs 8 bytes of workspace");
     hipStream_t st = (hipStream_t)stream;
+    if (Ainv && K <= BVLS_FAST_K && workspace_bytes >= adx_bvls_workspace_bytes(K)) {
+        const int64_t sets = (int64_t)1 << K;
+        double* table = reinterpret_cast<double*>(static_cast<char*>(workspace) + 256);
+        int* sing = reinterpret_cast<int*>(table + sets * K * K);
+        bvls_subset_inv_kernel<<<(unsigned)sets, 64, 0, st>>>(A, K, table, sing);
+        const unsigned blocks = (unsigned)ceil_div(nprob, 256);
+        if (K <= 4)
+            bvls_thread_kernel<4><<<blocks, 256, 0, st>>>(A, Ainv, B, X, nprob, K, lo, hi, table, sing);
+        else if (K <= 8)
+            bvls_thread_kernel<8><<<blocks, 256, 0, st>>>(A, Ainv, B, X, nprob, K, lo, hi, table, sing);
+        else
+            bvls_thread_kernel<12><<<blocks, 256, 0, st>>>(A, Ainv, B, X, nprob, K, lo, hi, table, sing);
+        ADX_LAUNCH_CHECK("adx_bvls_batch");
+        return 0;
+    }
     auto* q = static_cast<unsigned long long*>(workspace);
     zero_u64_kernel<<<1, 1, 0, st>>>(q);
     const size_t lds = sizeof(double) * (size_t)BvlsLds::doubles(K) + sizeof(int) * (size_t)(3 * K + 4);

@@ -17,8 +17,9 @@
 //        genotype operand comes straight from the EM kernel's per-lane 16-byte
 //        row loads (lane (g, c): byte row 16t+c, samples s0+16g .. s0+16g+15).
 //   xty: A^T (L x samples) += Y^T (L x 4 SNPs) . X (4 SNPs x samples).  A
-//        64-row x 64-sample genotype tile is staged in LDS so that lane (g, c)
-//        reads field g of byte [row s][sample c] for the k-step of byte row s.
+//        64-row x 64-sample genotype tile and its 256 Y rows are staged in
+//        LDS so that lane (g, c) reads field g of byte [row s][sample c] and
+//        Y[4s + g][li] for the k-step of byte row s.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -115,54 +116,81 @@ __global__ __launch_bounds__(256) void svd_xw_kernel(const uint8_t* __restrict__
     }
 }
 
-// Workgroup: 64 samples (16 per wave), one SNP range; byte-row chunks of 64
-// rows staged in LDS.
+// Workgroup: 256 samples (64 per wave, as 4 sub-tiles of 16), one SNP range;
+// chunks of 64 byte rows (256 SNPs) staged in LDS together with their Y rows
+// (transposed, li-major, row stride YS = 260 floats so that lane (g, c)
+// reading Y[4s + g][16 lt + c] hits bank 4c + g: all 64 lanes distinct) and
+// their 2 f values.  The inner loop reads LDS only, and each Y value read
+// feeds the 4 sub-tiles of the wave.
+constexpr int XTY_SUB = 4;
+
 template <int LT>
 __global__ __launch_bounds__(256) void svd_xty_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
                                                       int64_t N, const float* __restrict__ f,
                                                       const float* __restrict__ Y, int L, int64_t rows_per_split,
                                                       float* __restrict__ A, float* __restrict__ part) {
+    constexpr int YS = 260;
+    constexpr int WS = 64 * XTY_SUB;          // samples per workgroup
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
     const int cl = lane & 15;
-    const int64_t s0 = (int64_t)blockIdx.x * 64;
+    const int64_t s0 = (int64_t)blockIdx.x * WS;
     const int64_t Mb = (M + 3) >> 2;
     const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
     const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
-    __shared__ __attribute__((aligned(16))) uint8_t tile[64][64];
+    __shared__ __attribute__((aligned(16))) uint8_t tile[64][WS];
+    __shared__ float sY[16 * LT][YS];
+    __shared__ float s2f[256];
 
-    floatx4 acc[LT];
+    floatx4 acc[XTY_SUB][LT];
 #pragma unroll
-    for (int lt = 0; lt < LT; ++lt) acc[lt] = floatx4{0, 0, 0, 0};
-    const int64_t smp = s0 + 16 * wave + cl;   // this lane's sample (B-operand column)
+    for (int q = 0; q < XTY_SUB; ++q)
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) acc[q][lt] = floatx4{0, 0, 0, 0};
 
     for (int64_t rb = r0; rb < r1; rb += 64) {
-        {   // stage 64 byte rows x 64 samples: thread -> row tid/4, 16 bytes at 16*(tid%4)
-            const int r = threadIdx.x >> 2, c = 16 * (threadIdx.x & 3);
+        const int nrows = (int)(r1 - rb < 64 ? r1 - rb : 64);
+        const int64_t snp0 = 4 * rb;
+        // stage 64 byte rows x WS samples, 16 bytes per thread and step
+        for (int e = threadIdx.x; e < 64 * (WS / 16); e += 256) {
+            const int r = e / (WS / 16), c = 16 * (e - r * (WS / 16));
             uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-            if (rb + r < r1) v = *reinterpret_cast<const uint4*>(G + (rb + r) * ldg + s0 + c);
+            if (r < nrows && s0 + c < ldg) v = *reinterpret_cast<const uint4*>(G + (rb + r) * ldg + s0 + c);
             *reinterpret_cast<uint4*>(&tile[r][c]) = v;
         }
+        {   // the chunk's 256 SNPs: 2 f and Y rows (zero beyond M and L)
+            const int64_t snp = snp0 + threadIdx.x;
+            s2f[threadIdx.x] = snp < M ? 2.0f * f[snp] : 0.0f;
+            for (int e = threadIdx.x; e < 256 * 16 * LT; e += 256) {
+                const int sl = e / (16 * LT), li = e - sl * (16 * LT);
+                const int64_t sn = snp0 + sl;
+                sY[li][sl] = (sn < M && li < L) ? Y[sn * L + li] : 0.0f;
+            }
+        }
         __syncthreads();
-        const int nrows = (int)(r1 - rb < 64 ? r1 - rb : 64);
         for (int s = 0; s < nrows; ++s) {
-            const int64_t snp = 4 * (rb + s) + g;
-            uint32_t code = (tile[s][16 * wave + cl] >> (2 * g)) & 3u;
-            float twof = 0.0f;
-            if (snp < M) twof = 2.0f * f[snp];
-            else code = 3u;
-            const float x = centred(code, twof);
+            const int sl = 4 * s + g;
+            const bool live = snp0 + sl < M;
+            const float twof = s2f[sl];
+            float yv[LT];
 #pragma unroll
-            for (int lt = 0; lt < LT; ++lt) {
-                const int li = 16 * lt + cl;
-                const float yv = (snp < M && li < L) ? Y[snp * L + li] : 0.0f;
-                acc[lt] = Mma16<float>::run(yv, x, acc[lt]);
+            for (int lt = 0; lt < LT; ++lt) yv[lt] = sY[16 * lt + cl][sl];
+#pragma unroll
+            for (int q = 0; q < XTY_SUB; ++q) {
+                uint32_t code = (tile[s][64 * wave + 16 * q + cl] >> (2 * g)) & 3u;
+                if (!live) code = 3u;
+                const float x = centred(code, twof);
+#pragma unroll
+                for (int lt = 0; lt < LT; ++lt) acc[q][lt] = Mma16<float>::run(yv[lt], x, acc[q][lt]);
             }
         }
         __syncthreads();
     }
-    if (smp < N) {
+#pragma unroll
+    for (int q = 0; q < XTY_SUB; ++q) {
+        const int64_t smp = s0 + 64 * wave + 16 * q + cl;   // B-operand column of sub-tile q
+        if (smp >= N) continue;
 #pragma unroll
         for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
@@ -170,9 +198,9 @@ __global__ __launch_bounds__(256) void svd_xty_kernel(const uint8_t* __restrict_
                 const int li = 16 * lt + 4 * g + j;
                 if (li < L) {
                     if (part)   // this SNP split's slice, summed in order by svd_sum_kernel
-                        part[((int64_t)blockIdx.y * N + smp) * L + li] = acc[lt][j];
+                        part[((int64_t)blockIdx.y * N + smp) * L + li] = acc[q][lt][j];
                     else
-                        atomicAdd(&A[smp * L + li], acc[lt][j]);
+                        atomicAdd(&A[smp * L + li], acc[q][lt][j]);
                 }
             }
     }
@@ -206,7 +234,7 @@ static SvdGrid xw_grid(int64_t M, int64_t N) {
 
 static SvdGrid xty_grid(int64_t M, int64_t N) {
     const int64_t Mb = ceil_div(M, 4);
-    const int64_t nbx = ceil_div(N, 64);
+    const int64_t nbx = ceil_div(N, 64 * XTY_SUB);
     int64_t s = ceil_div(2048, nbx);
     const int64_t max_s = ceil_div(Mb, 64);
     s = s > max_s ? max_s : (s < 1 ? 1 : s);

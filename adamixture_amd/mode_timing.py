@@ -1,5 +1,6 @@
-"""Timing of the projection, supervised and cross-validation modes on a
-synthetic workload (``python -m adamixture_amd.mode_timing --mode cv``).
+"""Timing of the projection, supervised and cross-validation modes and of
+the SVD + ALS initialisation on a synthetic workload
+(``python -m adamixture_amd.mode_timing --mode cv``).
 
 Prints one JSON line per mode: wall seconds per iteration (Adam-EM modes,
 iterations after the priming step, logL checks included) or per fold (CV:
@@ -88,9 +89,27 @@ def time_cv(G, P0, Q0, w, dev, folds: int, timed_folds: int) -> dict:
             "s_per_fold": round(dt / timed_folds, 4), "fold_deviance_sums": sums}
 
 
+def time_init(G, w, dev) -> dict:
+    """Randomized SVD (power 5, tol 0.1) and ALS (to tol 1e-4) as the plain
+    method initialises; the second of two runs is reported."""
+    from .als_gpu import ALS_gpu
+    from .genotypes import allele_frequencies
+    from .svd_gpu import SVD_gpu
+    f = allele_frequencies(G)
+    out = {}
+    for _ in range(2):
+        t0 = _sync(dev)
+        U, S, V = SVD_gpu(G, w.N, w.M, f, w.K, 42, 5, 0.1, 0, dev)
+        t1 = _sync(dev)
+        ALS_gpu(U, S, V, f, 42, w.M, w.K, 1000, 1e-4, dev)
+        t2 = _sync(dev)
+        out = {"mode": "init", "svd_s": round(t1 - t0, 4), "als_s": round(t2 - t1, 4)}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--mode", choices=["projection", "supervised", "cv", "all"], default="all")
+    ap.add_argument("--mode", choices=["projection", "supervised", "cv", "init", "all"], default="all")
     ap.add_argument("--workload", default="biobank", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="bf16", choices=["f64", "f32", "bf16"])
     ap.add_argument("--iters", type=int, default=20)
@@ -107,6 +126,8 @@ def main(argv=None) -> int:
         print(json.dumps({**base, **time_supervised(G, P0, Q0, w, dev, args.precision, args.iters)}), flush=True)
     if args.mode in ("cv", "all"):
         print(json.dumps({**base, **time_cv(G, P0, Q0, w, dev, args.folds, args.timed_folds)}), flush=True)
+    if args.mode in ("init", "all"):
+        print(json.dumps({**base, **time_init(G, w, dev)}), flush=True)
     return 0
 
 

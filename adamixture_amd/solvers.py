@@ -68,7 +68,25 @@ def batch_bvls_bpp(A: torch.Tensor, B: torch.Tensor, lower: float, upper: float,
     B = _f64c(B, dev)
     Ainv = torch.linalg.inv(A).contiguous() if start == "inverse" else None
     X = torch.empty((M, K), dtype=torch.float64, device=dev)
+    if Ainv is not None:
+        nb = _lib.load().adx_bvls_workspace_bytes(K)
+        ws = _bvls_workspace(dev, nb)
+        _lib.call("adx_bvls_batch", A.data_ptr(), Ainv.data_ptr(), B.data_ptr(), X.data_ptr(), M, K, lower, upper,
+                  ws.data_ptr(), nb, _lib.stream_handle(dev))
+        return X
     q = _queue(dev)
-    _lib.call("adx_bvls_batch", A.data_ptr(), _lib.ptr(Ainv), B.data_ptr(), X.data_ptr(), M, K, lower, upper,
+    _lib.call("adx_bvls_batch", A.data_ptr(), None, B.data_ptr(), X.data_ptr(), M, K, lower, upper,
               q.data_ptr(), 8, _lib.stream_handle(dev))
     return X
+
+
+_BVLS_WS: dict = {}
+
+
+def _bvls_workspace(dev, nbytes: int) -> torch.Tensor:
+    """Per-device workspace of the shared-A BVLS path (the subset-inverse
+    table); the first 8 bytes double as the work queue of the wavefront path."""
+    w = _BVLS_WS.get(dev)
+    if w is None or w.numel() < nbytes:
+        w = _BVLS_WS[dev] = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+    return w

@@ -237,7 +237,12 @@ int adx_sqp_solve(int simplex, const double *H, const double *r, const double *X
  * least squares min 1/2 x'Ax - b'x, lo <= x <= hi by block principal pivoting.
  * With Ainv = inv(A) the pivoting starts from X0 = inv(A) b as the CUDA op
  * does; with Ainv = NULL it starts from the Gauss-Jordan solution as the
- * Cython batch_bvls_bpp (bvls.pyx:130-226).  workspace: >= 8 bytes. */
+ * Cython batch_bvls_bpp (bvls.pyx:130-226).  workspace: >= 8 bytes; with
+ * Ainv given, K <= 12 and adx_bvls_workspace_bytes(K) bytes, the inverses of
+ * all 2^K principal submatrices of A are formed once and each problem runs on
+ * one thread (same pivoting rules; the free-set solve is inv(A_FF) b_F,
+ * equal to the Gauss-Jordan solution up to rounding). */
+int64_t adx_bvls_workspace_bytes(int K);
 int adx_bvls_batch(const double *A, const double *Ainv, const double *B, double *X, int64_t nprob, int K,
                    double lo, double hi, void *workspace, int64_t workspace_bytes, void *stream);
 

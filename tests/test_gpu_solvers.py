@@ -71,7 +71,7 @@ def test_sqp_from_real_gradients(cuda):
     np.testing.assert_allclose(got, want, atol=1e-9)
 
 
-@pytest.mark.parametrize("K", [1, 3, 8, 16, 33])
+@pytest.mark.parametrize("K", [1, 3, 8, 12, 16, 33])
 @pytest.mark.parametrize("start", ["inverse", "solve"])
 def test_bvls_matches_oracle(cuda, K, start):
     rng = np.random.default_rng(K)
@@ -83,3 +83,29 @@ def test_bvls_matches_oracle(cuda, K, start):
                                  start=start).cpu().numpy()
     np.testing.assert_allclose(got, want, atol=1e-9)
     assert got.min() >= -1e-12 and got.max() <= 1 + 1e-12
+
+
+@pytest.mark.parametrize("K", [4, 10, 12])
+def test_bvls_thread_path_matches_wavefront_path(cuda, K):
+    """The shared-A path (subset-inverse table, one thread per problem) and
+    the wavefront path (Gauss-Jordan per free set) on ALS-like problems where
+    many variables end on a bound: the same active sets, solutions equal to
+    rounding."""
+    from adamixture_amd import _lib
+    rng = np.random.default_rng(100 + K)
+    Qm = rng.dirichlet(np.full(K, 0.3), size=2000)
+    A = Qm.T @ Qm
+    B = rng.normal(0.5, 0.8, size=(20000, K)) @ A
+    Ad, Bd = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda)
+    fast = solvers.batch_bvls_bpp(Ad, Bd, 0.0, 1.0).cpu().numpy()
+    Ainv = torch.linalg.inv(Ad).contiguous()
+    slow = torch.empty_like(Bd)
+    q = torch.zeros(1, dtype=torch.int64, device=cuda)
+    _lib.call("adx_bvls_batch", Ad.data_ptr(), Ainv.data_ptr(), Bd.data_ptr(), slow.data_ptr(), B.shape[0], K, 0.0,
+              1.0, q.data_ptr(), 8, _lib.stream_handle(cuda))
+    slow = slow.cpu().numpy()
+    at_bound = np.mean((fast <= 1e-12) | (fast >= 1 - 1e-12))
+    print(f"K={K}: {at_bound:.2f} of the entries on a bound; max |fast - slow| {np.abs(fast - slow).max():.2e}")
+    assert at_bound > 0.2
+    np.testing.assert_array_equal(fast <= 1e-12, slow <= 1e-12)
+    np.testing.assert_allclose(fast, slow, rtol=0, atol=1e-10)

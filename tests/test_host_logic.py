@@ -12,7 +12,7 @@ from adamixture_amd import utils
 
 def test_manage_gpu_memory_places_or_raises(monkeypatch):
     M, N, K = 500_000, 100_000, 10
-    G = torch.zeros(((M + 3) // 4, N), dtype=torch.uint8)          # host tensor, never moved here
+    G = torch.zeros(1, dtype=torch.uint8).expand((M + 3) // 4, N)   # host tensor of the full shape, no storage
     placed = []
     monkeypatch.setattr(utils, "as_packed", lambda g, m, dev: placed.append((m, str(dev))) or "device-copy")
     need = ((M + 3) // 4) * ((N + 63) // 64 * 64) + utils.estimate_gpu_bytes(M, N, K, "adamem")
