@@ -1,25 +1,32 @@
-// BF16 form of the fused decode + EM-statistics pass (FP32 accumulation).
+// Mixed-precision (BF16 / FP16 operand) form of the fused decode +
+// EM-statistics pass, FP32 accumulation.
 //
 // Same tiling as em_pass_kernel (adx_em.hip): a workgroup owns 64*CW SNPs and
 // a sample strip, lane (g, c) streams 16 genotype bytes of one byte row per
 // 64-sample sub-tile, A and B live in MFMA accumulators for the strip and T
-// is reduced per sub-tile.  The contractions run on 16x16x32 BF16 MFMA:
+// is reduced per sub-tile.  The contractions run on 16x16x32 MFMA:
 //
-//   H^T  = Qx . Px^T with Qx = [Q_hi, Q_hi, Q_lo], Px = [P_hi, P_lo, P_hi]
-//          (three-term split: |H - H_exact| <~ 3 * 2^-16 |H|); one MFMA for
-//          K <= 10, two for K <= 16
-//   A^T += (Qt_hi + Qt_lo) . a^T,  B^T += (Qt_hi + Qt_lo) . b^T
-//          k = 32 samples (two 16-sample row blocks), a, b rounded to BF16
-//   T   += [a b] . [P_hi ; 1 - P_hi]   k = 16 SNPs x {a, b}: a and b of one
-//          genotype are rounded into one word (bf16 pair), the 16 x 16 word
-//          tile is transposed through LDS (one 32-bit write per genotype, one
+//   H^T  = Qx . Px^T (BF16) with Qx = 8 [Q_hi, Q_hi, Q_lo], Px = [P_hi, P_lo,
+//          P_hi] (three-term split: |H - H_exact| <~ 3 * 2^-16 |H|); one
+//          MFMA for K <= 10, two for K <= 16.  The factor 8 (exact) gives
+//          h8 = 8 H.
+//   A^T += (Qt_hi + Qt_lo) . a'^T,  B^T += (Qt_hi + Qt_lo) . b'^T   (FP16)
+//          k = 32 samples (two 16-sample row blocks); Qt = 256 Q split into
+//          two FP16 terms, a' = a / 8 and b' = b / 8 rounded to FP16.
+//   T   += [a' b'] . [P ; 1 - P]   (FP16)  k = 16 SNPs x {a', b'}: a' and b'
+//          of one genotype share one word (f16 pair), the 16 x 16 word tile
+//          is transposed through LDS (one 32-bit write per genotype, one
 //          128-bit read per lane), and the colsum(b) term of em_batch_math
-//          is the (1 - P) half.  The same words, permuted, are the B
-//          operands of the A and B contractions.
+//          is the (1 - P) half.  P here is P_hi + P_lo rounded to FP16.
 //
-// a = g/H and b = (2-g)/(1-H) are formed in FP32 from the FP32 accumulator,
-// with one reciprocal of H(1-H).  The operand arrays Px, Qx, Qt are rebuilt
-// from the FP64 P, Q by adx_em_prep_bf16 before every pass.
+// Per genotype: hc8 = clamp(h8, 8e-5, 8 (1 - 1e-5)) (H clamped to [1e-5,
+// 1 - 1e-5], em_adam_gpu.py:62-72), 1 / (8 H) = rcp(hc8), 1 / (8 (1 - H)) =
+// rcp(8 - hc8) in FP32, both rounded to FP16 (at most 12500, in range) and
+// multiplied, two genotypes per packed FP16 multiply, by g and 2 - g (0 for
+// a missing genotype) looked up from the 2-bit codes with byte permutes.
+// The slice reductions undo the power-of-two scales exactly (T x 8,
+// A and B x 1/32).  The operand arrays Px, Qx, Qt are rebuilt from the FP64
+// P, Q by adx_em_prep_bf16 before every pass.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -29,39 +36,36 @@
 
 namespace adx {
 
-// Per-byte genotype weights of four genotypes at once: w holds one 2-bit
-// code per byte (bits above cleared); returns ga = g and gb = 2 - g per byte,
-// both 0 where the code is 3 (missing).
-__device__ __forceinline__ void code_weights(uint32_t w, uint32_t& ga, uint32_t& gb) {
-    const uint32_t m = (w >> 1) & w & 0x01010101u;   // 1 in missing bytes
-    ga = w ^ (m | (m << 1));                          // 3 -> 0
-    gb = (0x02020202u ^ (m << 1)) - ga;               // 2 - g, missing -> 0 (no borrows)
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// 16x16x32 FP16 MFMA, FP32 accumulation; operand and accumulator layouts as
+// mma_bf16.
+__device__ __forceinline__ floatx4 mma_f16(f16x8 a, f16x8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-// a = ga / H and b = gb / (1 - H) with H clamped to [1e-5, 1 - 1e-5]
-// (em_adam_gpu.py:62-72), one reciprocal of H (1 - H).
-__device__ __forceinline__ void em_terms_f32(float h, float ga, float gb, float& a, float& b) {
-    const float hc = __builtin_amdgcn_fmed3f(h, 1e-5f, 1.0f - 1e-5f);
-    const float u = 1.0f - hc;
-    const float r = __builtin_amdgcn_rcpf(hc * u);
-    a = ga * (u * r);
-    b = gb * (hc * r);
-}
+// High bytes of the FP16 genotype weights indexed by the 2-bit code (byte c
+// of the word; the low bytes are 0): g = {0, 1, 2, -} and 2 - g = {2, 1, 0, -},
+// 0 for the missing code 3.  FP16 1.0 = 0x3C00, 2.0 = 0x4000.
+constexpr uint32_t kWeightA = 0x00403C00u;
+constexpr uint32_t kWeightB = 0x00003C40u;
 
-template <int J>
-__device__ __forceinline__ float byte_to_float(uint32_t w) {
-    return (float)((w >> (8 * J)) & 0xFFu);
+// FP16 pairs (a'_j, a'_{j+1}) and (b'_j, b'_{j+1}) of two genotypes with codes
+// in bytes j, j+1 of cw (sel = byte selector 0x01040004 for j = 0, 0x03040204
+// for j = 2) and FP32 reciprocals ra = 1 / (8 H), rb = 1 / (8 (1 - H)).
+__device__ __forceinline__ void weight_pair(uint32_t cw, uint32_t sel, float ra0, float ra1, float rb0, float rb1,
+                                            uint32_t& a, uint32_t& b) {
+    // byte selector [4, c_j, 4, c_j+1]: selector 4 picks the zero first operand
+    const uint32_t s = __builtin_amdgcn_perm(0x04040404u, cw, sel);
+    const f16x2 ga = __builtin_bit_cast(f16x2, __builtin_amdgcn_perm(0u, kWeightA, s));
+    const f16x2 gb = __builtin_bit_cast(f16x2, __builtin_amdgcn_perm(0u, kWeightB, s));
+    a = __builtin_bit_cast(uint32_t, ga * f16x2{(_Float16)ra0, (_Float16)ra1});
+    b = __builtin_bit_cast(uint32_t, gb * f16x2{(_Float16)rb0, (_Float16)rb1});
 }
 
 __device__ __forceinline__ bf16x8 load_bf16x8(const __bf16* p) {
     return *reinterpret_cast<const bf16x8*>(p);
-}
-
-// Two FP32 values rounded to BF16 and packed into one word (lo = x, hi = y).
-__device__ __forceinline__ uint32_t pack_bf16x2(float x, float y) {
-    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-    const bf16x2 v = bf16x2{(__bf16)x, (__bf16)y};
-    return __builtin_bit_cast(uint32_t, v);
 }
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -69,9 +73,8 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 template <int KH, int CW>
 __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
-    const __bf16* __restrict__ Qx, const __bf16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
-    float* __restrict__ Aacc, float* __restrict__ Bacc, float* __restrict__ Tacc, float* __restrict__ Tpart,
-    float* __restrict__ ABpart) {
+    const __bf16* __restrict__ Qx, const _Float16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
+    float* __restrict__ Tpart, float* __restrict__ ABpart) {
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
     constexpr int LX = 32 * KH;  // row length of Px / Qx
     constexpr int TS = 20;       // row stride (words) of the per-wave transpose tile
@@ -85,7 +88,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
 
-    // Per wave: word [sample row r][SNP column c] = bf16 a | bf16 b << 16.
+    // Per wave: word [sample row r][SNP column c] = f16 a' | f16 b' << 16.
     __shared__ __attribute__((aligned(16))) uint32_t s_tile[4][16 * TS];
     // Per wave: T of the 64 samples of the current row block over the wave's
     // SNPs; each (sample, k) is written once per wave and row block, and the
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
 
     const bf16x8 zero8 = bf16x8{};
     bf16x8 pH[CW][KH];
-    bf16x8 pT[CW];
+    f16x8 pT[CW];
 #pragma unroll
     for (int cb = 0; cb < CW; ++cb) {
         const int t = cb >> 2, f = cb & 3;
@@ -103,17 +106,18 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
 #pragma unroll
         for (int kh = 0; kh < KH; ++kh) pH[cb][kh] = snp < M ? load_bf16x8(Px + snp * LX + 32 * kh + 8 * g) : zero8;
         // T B-operand: k = 8g + i <-> (SNP column 4g + i/2, P for even i, 1 - P
-        // for odd i), matching the (a, b) pairs of the tile words; column kp = cl.
-        bf16x8 v;
+        // for odd i), matching the (a', b') pairs of the tile words; column
+        // kp = cl.  P = P_hi + P_lo from the Px row.
+        f16x8 v;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int64_t s2 = snp_w + 64 * t + 4 * (4 * g + (i >> 1)) + f;
             float x = 0.0f;
             if (s2 < M && cl < K) {
-                const float p = (float)Px[s2 * LX + cl];
+                const float p = (float)Px[s2 * LX + cl] + (float)Px[s2 * LX + K + cl];
                 x = (i & 1) ? 1.0f - p : p;
             }
-            v[i] = (__bf16)x;
+            v[i] = (_Float16)x;
         }
         pT[cb] = v;
     }
@@ -125,95 +129,137 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         accB[cb] = floatx4{0, 0, 0, 0};
     }
     uint32_t* tile = s_tile[wave];
+
+    // Genotype words: lane (g, cl) reads byte row row_w + 16t + cl (clamped
+    // into the matrix), and gor[t] marks as missing the SNP fields past M
+    // in the last byte row and whole rows past it.
+    const uint8_t* grow[CW / 4];
+    uint32_t gor[CW / 4];
+#pragma unroll
+    for (int t = 0; t < CW / 4; ++t) {
+        const int64_t row = row_w + 16 * t + cl;
+        const int64_t nf = M - 4 * row;
+        grow[t] = G + (row < Mb ? row : Mb - 1) * ldg + 16 * g;
+        gor[t] = row >= Mb ? 0xFFFFFFFFu : nf < 4 ? ((0xFFu << (2 * nf)) & 0xFFu) * 0x01010101u : 0u;
+    }
+    // The Q operands of a 64-sample sub-tile are staged once per workgroup:
+    // thread t copies Qt chunk (row t >> 3, samples 8 (t & 7) .. + 7) and KH
+    // Qx chunks (16 bytes each) into LDS in lane-fragment order,
+    //   s_qt[(2 tp + hl) 64 + lane]          = Qt row (hl ? 16 + cl : cl), samples 16g + 8tp .. + 7
+    //   s_qx[((2 tp + rr) KH + kh) 64 + lane] = Qx row 16 (cl >> 2) + 4 (2tp + rr) + (cl & 3),
+    //                                          columns 32 kh + 8 g .. + 7 (zero past N)
+    // so each wave's 128-bit operand reads hit consecutive addresses.
+    __shared__ u32x4 s_qt[4 * 64];
+    __shared__ u32x4 s_qx[4 * KH * 64];
+    const int tid = threadIdx.x;
+    const _Float16* qt_src = Qt + (int64_t)(tid >> 3) * ldqt + 8 * (tid & 7);
+    const int qt_dst = (2 * (tid & 1) + (tid >> 7)) * 64 + 16 * ((tid & 7) >> 1) + ((tid >> 3) & 15);
+    int qx_row[KH], qx_col[KH], qx_dst[KH];
+#pragma unroll
+    for (int j = 0; j < KH; ++j) {
+        const int idx = j * 256 + tid;
+        const int q = idx / (4 * KH), ch = idx % (4 * KH);
+        qx_row[j] = q;
+        qx_col[j] = 8 * ch;
+        qx_dst[j] = ((((q >> 2) & 3) * KH + (ch >> 2)) * 64) + 16 * (ch & 3) + (((q >> 4) << 2) | (q & 3));
+    }
+    const u32x4 zero4 = u32x4{0, 0, 0, 0};
+    u32x4 qn_t, qn_x[KH];
+    uint32_t gn[CW / 4][4];
+    auto load_next = [&](int64_t s) {
+        qn_t = *reinterpret_cast<const u32x4*>(qt_src + s);
+#pragma unroll
+        for (int j = 0; j < KH; ++j) {
+            const int64_t r = s + qx_row[j];
+            const u32x4 v = *reinterpret_cast<const u32x4*>(Qx + (r < N ? r : N - 1) * LX + qx_col[j]);
+            qn_x[j] = r < N ? v : zero4;
+        }
+#pragma unroll
+        for (int t = 0; t < CW / 4; ++t) {
+            const uint4 v = *reinterpret_cast<const uint4*>(grow[t] + s);
+            gn[t][0] = v.x;
+            gn[t][1] = v.y;
+            gn[t][2] = v.z;
+            gn[t][3] = v.w;
+        }
+    };
+    auto stage_next = [&](uint32_t (&gw)[CW / 4][4]) {
+        s_qt[qt_dst] = qn_t;
+#pragma unroll
+        for (int j = 0; j < KH; ++j) s_qx[qx_dst[j]] = qn_x[j];
+#pragma unroll
+        for (int t = 0; t < CW / 4; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) gw[t][r] = gn[t][r] | gor[t];
+    };
+    uint32_t gw[CW / 4][4];
+    load_next(n0);
+    stage_next(gw);
     __syncthreads();
 
     for (int64_t s0 = n0; s0 < n1; s0 += 64) {
-        uint32_t gw[CW / 4][4];
-#pragma unroll
-        for (int t = 0; t < CW / 4; ++t) {
-            const int64_t row = row_w + 16 * t + cl;
-            if (row < Mb) {
-                const uint4 v = *reinterpret_cast<const uint4*>(G + row * ldg + s0 + 16 * g);
-                gw[t][0] = v.x;
-                gw[t][1] = v.y;
-                gw[t][2] = v.z;
-                gw[t][3] = v.w;
-                const int64_t nf = M - 4 * row;
-                if (nf < 4) {
-                    const uint32_t pm = ((0xFFu << (2 * nf)) & 0xFFu) * 0x01010101u;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) gw[t][r] |= pm;
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) gw[t][r] = 0xFFFFFFFFu;
-            }
-        }
+        // The next sub-tile's genotype words and Q chunks are in flight
+        // during this sub-tile's arithmetic (the last sub-tile reloads its
+        // own, unused: the loads stay unconditional).
+        load_next(s0 + 64 < n1 ? s0 + 64 : s0);
 
         // Row-block pairs: accumulator element j of lane group g in row block
         // rw is sample s0 + 16g + 4rw + j; a pair (rw = 2tp, 2tp+1) gives lane
         // (g, cl) the 8 consecutive samples s0 + 16g + 8tp + i, i = 0..7.
 #pragma unroll
         for (int tp = 0; tp < 2; ++tp) {
-            const int64_t sq = s0 + 16 * g + 8 * tp;  // first of the 8 samples (Qt rows are padded)
-            const bf16x8 qThi = load_bf16x8(Qt + (int64_t)cl * ldqt + sq);
-            const bf16x8 qTlo = load_bf16x8(Qt + (16 + (int64_t)cl) * ldqt + sq);
+            const f16x8 qThi = __builtin_bit_cast(f16x8, s_qt[(2 * tp) * 64 + lane]);
+            const f16x8 qTlo = __builtin_bit_cast(f16x8, s_qt[(2 * tp + 1) * 64 + lane]);
             bf16x8 qH[2][KH];
 #pragma unroll
-            for (int rr = 0; rr < 2; ++rr) {
-                const int64_t sH = s0 + 16 * (cl >> 2) + 4 * (2 * tp + rr) + (cl & 3);
+            for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
                 for (int kh = 0; kh < KH; ++kh)
-                    qH[rr][kh] = sH < N ? load_bf16x8(Qx + sH * LX + 32 * kh + 8 * g) : zero8;
-            }
+                    qH[rr][kh] = __builtin_bit_cast(bf16x8, s_qx[((2 * tp + rr) * KH + kh) * 64 + lane]);
             floatx4 accT[2] = {floatx4{0, 0, 0, 0}, floatx4{0, 0, 0, 0}};
 
 #pragma unroll
             for (int cb = 0; cb < CW; ++cb) {
                 const int t = cb >> 2, f = cb & 3;
-                uint32_t wab[2][4];
+                // wa[rr][p] / wb[rr][p]: f16 pairs of a' / b' for samples 2p, 2p + 1
+                // of the 4 in accumulator order
+                uint32_t wa[2][2], wb[2][2];
 #pragma unroll
                 for (int rr = 0; rr < 2; ++rr) {
                     floatx4 h = floatx4{0, 0, 0, 0};
 #pragma unroll
                     for (int kh = 0; kh < KH; ++kh) h = mma_bf16(qH[rr][kh], pH[cb][kh], h);
-                    uint32_t ga, gb;
-                    code_weights((gw[t][2 * tp + rr] >> (2 * f)) & 0x03030303u, ga, gb);
-                    const float fa[4] = {byte_to_float<0>(ga), byte_to_float<1>(ga), byte_to_float<2>(ga),
-                                         byte_to_float<3>(ga)};
-                    const float fb[4] = {byte_to_float<0>(gb), byte_to_float<1>(gb), byte_to_float<2>(gb),
-                                         byte_to_float<3>(gb)};
+                    const uint32_t cw = (gw[t][2 * tp + rr] >> (2 * f)) & 0x03030303u;
+                    float ra[4], rb[4];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        float a, b;
-                        em_terms_f32(h[j], fa[j], fb[j], a, b);
-                        wab[rr][j] = pack_bf16x2(a, b);
-                        tile[(4 * g + j) * TS + cl] = wab[rr][j];
+                        const float hc8 = __builtin_amdgcn_fmed3f(h[j], 8.0f * 1e-5f, 8.0f * (1.0f - 1e-5f));
+                        ra[j] = __builtin_amdgcn_rcpf(hc8);
+                        rb[j] = __builtin_amdgcn_rcpf(8.0f - hc8);
+                    }
+                    weight_pair(cw, 0x01040004u, ra[0], ra[1], rb[0], rb[1], wa[rr][0], wb[rr][0]);
+                    weight_pair(cw, 0x03040204u, ra[2], ra[3], rb[2], rb[3], wa[rr][1], wb[rr][1]);
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        tile[(4 * g + 2 * p) * TS + cl] = __builtin_amdgcn_perm(wb[rr][p], wa[rr][p], 0x05040100u);
+                        tile[(4 * g + 2 * p + 1) * TS + cl] = __builtin_amdgcn_perm(wb[rr][p], wa[rr][p], 0x07060302u);
                     }
                     // The tile is written and read by lanes of this wave only;
                     // LDS operations of a wave complete in order, and the
                     // wave barrier keeps the compiler from moving them.
                     __builtin_amdgcn_wave_barrier();
-                    // T (16 samples x 16 kp) += [a b] (16 x 32) . [P ; 1-P] (32 x 16)
-                    const bf16x8 tv = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(tile + cl * TS + 4 * g));
+                    // T (16 samples x 16 kp) += [a' b'] (16 x 32) . [P ; 1-P] (32 x 16)
+                    const f16x8 tv = __builtin_bit_cast(f16x8, *reinterpret_cast<const u32x4*>(tile + cl * TS + 4 * g));
                     __builtin_amdgcn_wave_barrier();
-                    accT[rr] = mma_bf16(tv, pT[cb], accT[rr]);
+                    accT[rr] = mma_f16(tv, pT[cb], accT[rr]);
                 }
                 // B operands of A^T, B^T: k = sample 8g + i (rr = i / 4, j = i % 4).
-                const u32x4 wa = u32x4{__builtin_amdgcn_perm(wab[0][1], wab[0][0], 0x05040100u),
-                                       __builtin_amdgcn_perm(wab[0][3], wab[0][2], 0x05040100u),
-                                       __builtin_amdgcn_perm(wab[1][1], wab[1][0], 0x05040100u),
-                                       __builtin_amdgcn_perm(wab[1][3], wab[1][2], 0x05040100u)};
-                const u32x4 wb = u32x4{__builtin_amdgcn_perm(wab[0][1], wab[0][0], 0x07060302u),
-                                       __builtin_amdgcn_perm(wab[0][3], wab[0][2], 0x07060302u),
-                                       __builtin_amdgcn_perm(wab[1][1], wab[1][0], 0x07060302u),
-                                       __builtin_amdgcn_perm(wab[1][3], wab[1][2], 0x07060302u)};
-                const bf16x8 aB = __builtin_bit_cast(bf16x8, wa);
-                const bf16x8 bB = __builtin_bit_cast(bf16x8, wb);
-                accA[cb] = mma_bf16(qThi, aB, accA[cb]);
-                accA[cb] = mma_bf16(qTlo, aB, accA[cb]);
-                accB[cb] = mma_bf16(qThi, bB, accB[cb]);
-                accB[cb] = mma_bf16(qTlo, bB, accB[cb]);
+                const f16x8 aB = __builtin_bit_cast(f16x8, u32x4{wa[0][0], wa[0][1], wa[1][0], wa[1][1]});
+                const f16x8 bB = __builtin_bit_cast(f16x8, u32x4{wb[0][0], wb[0][1], wb[1][0], wb[1][1]});
+                accA[cb] = mma_f16(qThi, aB, accA[cb]);
+                accA[cb] = mma_f16(qTlo, aB, accA[cb]);
+                accB[cb] = mma_f16(qThi, bB, accB[cb]);
+                accB[cb] = mma_f16(qTlo, bB, accB[cb]);
             }
 #pragma unroll
             for (int rr = 0; rr < 2; ++rr)
@@ -223,6 +269,8 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         }
 
         __syncthreads();
+        // every wave is past its reads of the staged Q chunks
+        stage_next(gw);
         // T of these 64 samples over this workgroup's SNPs: one plain store
         // into the workgroup's slice of the partial buffer (every (slice,
         // sample) is written exactly once), summed by tpart_reduce_kernel.
@@ -232,13 +280,9 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             const int kp = e & 15;
             const int64_t smp = s0 + (e >> 4);
             const int ex = (e >> 4) * 17 + kp;
-            if (kp < K && smp < N) {
-                const float v = (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
-                if (Tpart)
-                    Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = v;
-                else
-                    atomicAdd(&Tacc[smp * K + kp], v);
-            }
+            if (kp < K && smp < N)
+                Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] =
+                    (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
         }
         __syncthreads();
     }
@@ -254,25 +298,22 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
                 if (kp < K) {
                     // this sample split's slice of the A / B partial sums (each
                     // element written by one workgroup), summed in a fixed order
-                    if (ABpart) {
-                        const int64_t mk = M * (int64_t)K;
-                        ABpart[(int64_t)blockIdx.y * mk + snp * K + kp] = accA[cb][jj];
-                        ABpart[((int64_t)gridDim.y + blockIdx.y) * mk + snp * K + kp] = accB[cb][jj];
-                    } else {
-                        atomicAdd(&Aacc[snp * K + kp], accA[cb][jj]);
-                        atomicAdd(&Bacc[snp * K + kp], accB[cb][jj]);
-                    }
+                    const int64_t mk = M * (int64_t)K;
+                    ABpart[(int64_t)blockIdx.y * mk + snp * K + kp] = accA[cb][jj];
+                    ABpart[((int64_t)gridDim.y + blockIdx.y) * mk + snp * K + kp] = accB[cb][jj];
                 }
             }
         }
     }
 }
 
-// Operand arrays of the BF16 pass from the FP64 master P (M x K) and Q (N x K):
+// Operand arrays of the pass from the FP64 master P (M x K) and Q (N x K):
 //   Px[m]  = [P_hi(0..K), P_lo, P_hi, 0 ...]    (32*KH bf16)
-//   Qx[i]  = [Q_hi, Q_hi, Q_lo, 0 ...]          (32*KH bf16)
-//   Qt     = [hi rows 0..15 | lo rows 0..15] x ldqt, Qt[kp][i] = Q[i][kp]
-// with x_hi = bf16(x), x_lo = bf16(x - x_hi); zero outside the valid range.
+//   Qx[i]  = 8 [Q_hi, Q_hi, Q_lo, 0 ...]        (32*KH bf16)
+//   Qt     = [hi rows 0..15 | lo rows 0..15] x ldqt, Qt[kp][i] = 256 Q[i][kp]
+//            split into two FP16 terms
+// with x_hi = bf16(x), x_lo = bf16(x - x_hi) (FP16 for Qt); zero outside the
+// valid range.  The power-of-two factors are exact.
 template <int KH>
 __global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, __bf16* __restrict__ Px) {
     constexpr int LX = 32 * KH;
@@ -295,7 +336,7 @@ __global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, _
 
 template <int KH>
 __global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, __bf16* __restrict__ Qx,
-                               __bf16* __restrict__ Qt, int64_t ldqt) {
+                               _Float16* __restrict__ Qt, int64_t ldqt) {
     constexpr int LX = 32 * KH;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ldqt) return;
@@ -303,17 +344,20 @@ __global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, _
 #pragma unroll
     for (int e = 0; e < LX; ++e) row[e] = (__bf16)0.0f;
     for (int k = 0; k < 16; ++k) {
-        __bf16 hi = (__bf16)0.0f, lo = (__bf16)0.0f;
+        _Float16 thi = (_Float16)0.0f, tlo = (_Float16)0.0f;
         if (i < N && k < K) {
             const float x = (float)Q[i * K + k];
-            hi = (__bf16)x;
-            lo = (__bf16)(x - (float)hi);
+            const __bf16 hi = (__bf16)(8.0f * x);
+            const __bf16 lo = (__bf16)(8.0f * x - (float)hi);
             row[k] = hi;
             row[K + k] = hi;
             row[2 * K + k] = lo;
+            const float y = 256.0f * x;
+            thi = (_Float16)y;
+            tlo = (_Float16)(y - (float)thi);
         }
-        Qt[(int64_t)k * ldqt + i] = hi;
-        Qt[(int64_t)(16 + k) * ldqt + i] = lo;
+        Qt[(int64_t)k * ldqt + i] = thi;
+        Qt[(int64_t)(16 + k) * ldqt + i] = tlo;
     }
     if (i < N)
 #pragma unroll
@@ -340,9 +384,9 @@ static Bf16Grid bf16_grid(int64_t M, int64_t N) {
     return gd;
 }
 
-// T[e] += sum over the nparts slices of Tpart[p * nk + e] (also used for the
-// A and B slices, nk = M K).
-__global__ void tpart_reduce_kernel(const float* __restrict__ Tpart, int64_t nparts, int64_t nk,
+// T[e] += scale * (sum over the nparts slices of Tpart[p * nk + e]) (also
+// used for the A and B slices, nk = M K); scale is a power of two.
+__global__ void tpart_reduce_kernel(const float* __restrict__ Tpart, int64_t nparts, int64_t nk, float scale,
                                     float* __restrict__ T) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nk) return;
@@ -355,25 +399,27 @@ __global__ void tpart_reduce_kernel(const float* __restrict__ Tpart, int64_t npa
         s3 += Tpart[(p + 3) * nk + e];
     }
     for (; p < nparts; ++p) s0 += Tpart[p * nk + e];
-    T[e] += (s0 + s1) + (s2 + s3);
+    T[e] += scale * ((s0 + s1) + (s2 + s3));
 }
 
 template <int KH>
 static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
-                        int K, const __bf16* Px, const __bf16* Qx, const __bf16* Qt, int64_t ldqt, float* A,
+                        int K, const __bf16* Px, const __bf16* Qx, const _Float16* Qt, int64_t ldqt, float* A,
                         float* B, float* T, float* Tpart) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     float* ABpart = Tpart + gd.nbx * N * (int64_t)K;
     if (gd.cw == 8)
-        em_pass_bf16_kernel<KH, 8>
-            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart, ABpart);
+        em_pass_bf16_kernel<KH, 8><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, Tpart,
+                                                         ABpart);
     else
-        em_pass_bf16_kernel<KH, 4>
-            <<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, A, B, T, Tpart, ABpart);
+        em_pass_bf16_kernel<KH, 4><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, Tpart,
+                                                         ABpart);
+    // slices hold T / 8 (a' = a / 8) and 32 A, 32 B (Qt = 256 Q)
     const int64_t nk = N * K, mk = M * K;
-    tpart_reduce_kernel<<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, T);
-    tpart_reduce_kernel<<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart, gd.splits, mk, A);
-    tpart_reduce_kernel<<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart + gd.splits * mk, gd.splits, mk, B);
+    tpart_reduce_kernel<<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, 8.0f, T);
+    tpart_reduce_kernel<<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart, gd.splits, mk, 1.0f / 32.0f, A);
+    tpart_reduce_kernel<<<(unsigned)ceil_div(mk, 256), 256, 0, st>>>(ABpart + gd.splits * mk, gd.splits, mk,
+                                                                      1.0f / 32.0f, B);
 }
 
 // Byte offset of the partial-T slices in the operand buffer, and its size.
@@ -406,7 +452,7 @@ int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int
     const int kh = K <= 10 ? 1 : 2;
     __bf16* Px = static_cast<__bf16*>(operands);
     __bf16* Qx = Px + M * 32 * kh;
-    __bf16* Qt = Qx + N * 32 * kh;
+    _Float16* Qt = reinterpret_cast<_Float16*>(Qx + N * 32 * kh);
     hipStream_t st = (hipStream_t)stream;
     if (kh == 1) {
         if (M > 0) prep_px_kernel<1><<<(unsigned)ceil_div(M, 256), 256, 0, st>>>(P, M, K, Px);
@@ -433,7 +479,7 @@ int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
     const int kh = K <= 10 ? 1 : 2;
     const __bf16* Px = static_cast<const __bf16*>(operands);
     const __bf16* Qx = Px + M * 32 * kh;
-    const __bf16* Qt = Qx + N * 32 * kh;
+    const _Float16* Qt = reinterpret_cast<const _Float16*>(Qx + N * 32 * kh);
     float* Tpart = reinterpret_cast<float*>(static_cast<uint8_t*>(operands) + bf16_base_bytes(M, N, K));
     const Bf16Grid gd = bf16_grid(M, N);
     hipStream_t st = (hipStream_t)stream;

@@ -219,6 +219,22 @@ __global__ void probe_bf16_kernel(const float* __restrict__ A, const float* __re
     for (int j = 0; j < 4; ++j) D[l * 4 + j] = d[j];
 }
 
+// FP16 probe of the 16x16x32 FP16 MFMA, same operand and output convention
+// as the BF16 probe (values exact in fp16).
+__global__ void probe_f16_kernel(const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ D) {
+    typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+    const int l = threadIdx.x;
+    f16x8 a, b;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        a[i] = (_Float16)A[(l % 16) * 32 + 8 * (l / 16) + i];
+        b[i] = (_Float16)B[(8 * (l / 16) + i) * 16 + l % 16];
+    }
+    floatx4 d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, floatx4{0, 0, 0, 0}, 0, 0, 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) D[l * 4 + j] = d[j];
+}
+
 }  // namespace adx
 
 using namespace adx;
@@ -295,8 +311,11 @@ int adx_repack_padded(const uint8_t* src, int64_t ld_src, int64_t Mb, int64_t N,
 
 int adx_probe_mfma(int kind, const void* A, const void* B, void* D, void* stream) {
     const int raw = kind >= 2;
-    if (kind == 4) {
-        probe_bf16_kernel<<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D);
+    if (kind == 4 || kind == 5) {
+        if (kind == 4)
+            probe_bf16_kernel<<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D);
+        else
+            probe_f16_kernel<<<1, 64, 0, (hipStream_t)stream>>>((const float*)A, (const float*)B, (float*)D);
         ADX_LAUNCH_CHECK("adx_probe_mfma");
         return 0;
     }
@@ -306,7 +325,7 @@ int adx_probe_mfma(int kind, const void* A, const void* B, void* D, void* stream
         probe_mma_kernel<double><<<1, 64, 0, (hipStream_t)stream>>>((const double*)A, (const double*)B, (double*)D,
                                                                      raw);
     else
-        return fail_msg(1, "adx_probe_mfma: kind must be 0..4");
+        return fail_msg(1, "adx_probe_mfma: kind must be 0..5");
     ADX_LAUNCH_CHECK("adx_probe_mfma");
     return 0;
 }

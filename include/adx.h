@@ -251,9 +251,10 @@ int adx_bvls_batch(const double *A, const double *Ainv, const double *B, double 
 /* No reference counterpart: runs one 16x16x4 MFMA (kind 0 = FP32, 1 = FP64)
  * on row-major A (16x4) and B (4x16) with the lane mapping the EM kernels
  * assume, writing D (16x16) row-major.  kind 2 / 3 (FP32 / FP64) write the
- * raw accumulators instead, D[lane * 4 + j]; kind 4 runs one 16x16x32 BF16
- * MFMA on FP32 A (16x32) and B (32x16) holding bf16-exact values, writing raw
- * accumulators.  Used to pin the lane mappings. */
+ * raw accumulators instead, D[lane * 4 + j]; kind 4 (5) runs one 16x16x32
+ * BF16 (FP16) MFMA on FP32 A (16x32) and B (32x16) holding bf16-exact
+ * (fp16-exact) values, writing raw accumulators.  Used to pin the lane
+ * mappings. */
 int adx_probe_mfma(int kind, const void *A, const void *B, void *D, void *stream);
 
 /* --- learned Q warm start (project extension) ----------------------------
