@@ -10,8 +10,9 @@ Precision forms (``precision``):
     float64 on CUDA); H and the A/B/T contractions on FP64 MFMA.
   * ``"f32"``: H and the contractions on FP32 MFMA from FP32 copies of P and
     Q; P, Q, the Adam state and the log-likelihood stay FP64.
-  * ``"bf16"``: H (three-term hi/lo split) and the contractions on BF16 MFMA
-    with FP32 accumulation; P, Q, Adam state and log-likelihood FP64.
+  * ``"bf16"``: H (three-term hi/lo split) on BF16 MFMA, the EM weights in
+    FP16 and the A/B/T contractions on FP16 MFMA, FP32 accumulation; P, Q,
+    Adam state and log-likelihood FP64.
 """
 from __future__ import annotations
 

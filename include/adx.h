@@ -124,11 +124,14 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
                 const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
                 int flags, double *ll_out, void *workspace, int64_t workspace_bytes, void *stream);
 
-/* BF16 form of adx_em_pass (ADX_EM_STATS only, 1 <= K <= 16), FP32 A, B, T.
- * Same contract as adx_em_pass with dtype ADX_F32, but H and the A/B/T
- * contractions run on BF16 MFMA with FP32 accumulation from operand arrays
- * that adx_em_prep_bf16 builds from the FP64 P and Q (H from a three-term
- * hi/lo split; A and B from Q split in hi + lo; a, b rounded to BF16).
+/* Mixed-precision form of adx_em_pass (ADX_EM_STATS only, 1 <= K <= 16),
+ * FP32 A, B, T (dtype tag ADX_BF16).  Same contract as adx_em_pass with
+ * dtype ADX_F32, but the contractions run on 16x16x32 MFMA with FP32
+ * accumulation from operand arrays that adx_em_prep_bf16 builds from the
+ * FP64 P and Q: H on BF16 MFMA from a three-term hi/lo split; the EM weights
+ * a, b rounded to FP16 and the A, B, T contractions on FP16 MFMA (Q split in
+ * two FP16 terms).  Stated accuracy against the FP64 pass: 1e-3 relative on
+ * A, B, T.
  * The operand buffer also holds the per-workgroup partial sums of T
  * (ceil(M / (64 cw)) x N x K FP32, cw from adx_set_em_tiles) and the
  * per-sample-split partial sums of A and B (2 x splits x M x K), so its size
