@@ -218,17 +218,31 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
                     qH[rr][kh] = __builtin_bit_cast(bf16x8, s_qx[((2 * tp + rr) * KH + kh) * 64 + lane]);
             floatx4 accT[2] = {floatx4{0, 0, 0, 0}, floatx4{0, 0, 0, 0}};
 
+            // H of column block cb + 1 is issued before the weights of block
+            // cb are formed, so the MFMA latency overlaps that arithmetic
+            // (consuming H right after its MFMA stalled the wave).
+            auto hprod = [&](int cb, floatx4 (&h)[2]) {
+#pragma unroll
+                for (int rr = 0; rr < 2; ++rr) {
+                    h[rr] = floatx4{0, 0, 0, 0};
+#pragma unroll
+                    for (int kh = 0; kh < KH; ++kh) h[rr] = mma_bf16(qH[rr][kh], pH[cb][kh], h[rr]);
+                }
+            };
+            floatx4 hc[2], hn[2];
+            hprod(0, hc);
 #pragma unroll
             for (int cb = 0; cb < CW; ++cb) {
                 const int t = cb >> 2, f = cb & 3;
+                if (cb + 1 < CW) hprod(cb + 1, hn);
+                // keep the scheduler from sinking those MFMAs to their uses
+                __builtin_amdgcn_sched_barrier(0);
                 // wa[rr][p] / wb[rr][p]: f16 pairs of a' / b' for samples 2p, 2p + 1
                 // of the 4 in accumulator order
                 uint32_t wa[2][2], wb[2][2];
 #pragma unroll
                 for (int rr = 0; rr < 2; ++rr) {
-                    floatx4 h = floatx4{0, 0, 0, 0};
-#pragma unroll
-                    for (int kh = 0; kh < KH; ++kh) h = mma_bf16(qH[rr][kh], pH[cb][kh], h);
+                    const floatx4 h = hc[rr];
                     const uint32_t cw = (gw[t][2 * tp + rr] >> (2 * f)) & 0x03030303u;
                     float ra[4], rb[4];
 #pragma unroll
@@ -260,6 +274,10 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
                 accA[cb] = mma_f16(qTlo, aB, accA[cb]);
                 accB[cb] = mma_f16(qThi, bB, accB[cb]);
                 accB[cb] = mma_f16(qTlo, bB, accB[cb]);
+                if (cb + 1 < CW) {
+                    hc[0] = hn[0];
+                    hc[1] = hn[1];
+                }
             }
 #pragma unroll
             for (int rr = 0; rr < 2; ++rr)
