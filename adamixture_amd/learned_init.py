@@ -38,6 +38,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .als_gpu import _pinv_host
 from .genotypes import PackedGenotypes
 
 log = logging.getLogger(__name__)
@@ -101,7 +102,7 @@ def features(U2: torch.Tensor, S2: torch.Tensor, V2: torch.Tensor, f: torch.Tens
     X = torch.zeros((N, D_IN), dtype=torch.float32, device=dev)
     Qa = Q_als.to(dev, dt)
     X[:, :K] = Qa.to(torch.float32)
-    I_p = P @ torch.linalg.pinv(P.T @ P)
+    I_p = P @ _pinv_host(P.T @ P)
     Z2 = U2 * S2
     X[:, 16:16 + K] = (0.5 * (V2 @ (Z2.T @ I_p)) + f.reshape(1, -1) @ I_p).to(torch.float32)
     X[:, 32:32 + K] = (torch.log(Qa.clamp_min(1e-5)) / math.log(1e-5)).to(torch.float32)
@@ -129,7 +130,7 @@ def p_from_q(U2: torch.Tensor, S2: torch.Tensor, V2: torch.Tensor, f: torch.Tens
     V2 = V2.to(dev, dt)
     f = f.to(dev, dt)
     A_q = Q.T @ Q
-    I_q = Q @ torch.linalg.pinv(A_q)
+    I_q = Q @ _pinv_host(A_q)
     P_free = 0.5 * (Z2 @ (V2.T @ I_q)) + torch.outer(f, I_q.sum(dim=0))
     P = batch_bvls_bpp(A_q, P_free @ A_q, 0.0, 1.0)
     mapP(P)
@@ -145,7 +146,7 @@ def p_from_q_data(G: PackedGenotypes, f: torch.Tensor, Q: torch.Tensor) -> torch
     dt = torch.float64
     Q = Q.to(dt)
     A_q = Q.T @ Q
-    I_q = Q @ torch.linalg.pinv(A_q)
+    I_q = Q @ _pinv_host(A_q)
     Y = xw(G, f.to(torch.float32).contiguous(), I_q.to(torch.float32).contiguous()).to(dt)
     P_free = 0.5 * Y + torch.outer(f.to(dt), I_q.sum(dim=0))
     P = batch_bvls_bpp(A_q, P_free @ A_q, 0.0, 1.0)
