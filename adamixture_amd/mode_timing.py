@@ -1,5 +1,5 @@
-"""Timing of the projection, supervised and cross-validation modes and of
-the SVD + ALS initialisation on a synthetic workload
+"""Timing of the projection, supervised and cross-validation modes, of
+the SVD + ALS initialisation and of the file readers on synthetic workloads
 (``python -m adamixture_amd.mode_timing --mode cv``).
 
 Prints one JSON line per mode: wall seconds per iteration (Adam-EM modes,
@@ -107,9 +107,61 @@ def time_init(G, w, dev) -> dict:
     return out
 
 
+def _write_vcf(path, G: np.ndarray) -> None:
+    """Unpacked M x N codes as a minimal VCF (GT only; 3 = './.')."""
+    M, N = G.shape
+    gt = np.frombuffer(b"0/0\t0/1\t1/1\t./.\t", dtype=np.uint8).reshape(4, 4)
+    head = "##fileformat=VCFv4.3\n#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t"
+    head += "\t".join(f"s{i}" for i in range(N)) + "\n"
+    with open(path, "wb") as fh:
+        fh.write(head.encode())
+        for m0 in range(0, M, 1024):
+            blk = G[m0:m0 + 1024]
+            body = gt[blk].reshape(blk.shape[0], -1)
+            body[:, -1] = ord("\n")
+            for r in range(blk.shape[0]):
+                fh.write(b"1\t%d\t.\tA\tC\t.\t.\t.\tGT\t" % (m0 + r + 1))
+                fh.write(body[r].tobytes())
+
+
+def time_readers(dev, N: int = 10_000, M_bed: int = 100_000, M_vcf: int = 10_000) -> list[dict]:
+    """File -> device packed matrix: PLINK .bed (N x M_bed) and VCF text
+    (N x M_vcf) from one synthetic genotype matrix; the second of two reads
+    is reported (file in the page cache), with the read-back checked against
+    the source codes."""
+    import tempfile
+    from pathlib import Path
+    from .formats import read_vcf_gpu
+    from .genotypes import unpack_chunk
+    from .reader import read_bed_gpu, write_bed
+    from .synth import Workload
+    w = Workload("readers", N, M_bed, 4, "bn", 0.3, 0.01, fst=0.1, seed=11)
+    G, _, _ = make_genotypes(w, dev)
+    host = unpack_chunk(G, 0, M_bed).cpu().numpy()
+    out = []
+    with tempfile.TemporaryDirectory() as tmp:
+        base = Path(tmp) / "r"
+        write_bed(base, host)
+        vcf = Path(tmp) / "r.vcf"
+        _write_vcf(vcf, host[:M_vcf])
+        for kind, path, M, reader in (("bed", Path(f"{base}.bed"), M_bed, read_bed_gpu),
+                                      ("vcf", vcf, M_vcf, read_vcf_gpu)):
+            for _ in range(2):
+                t0 = _sync(dev)
+                Gr, n, m = reader(path, dev)
+                dt = _sync(dev) - t0
+            same = bool(np.array_equal(unpack_chunk(Gr, 0, M).cpu().numpy(), host[:M]))
+            size = path.stat().st_size
+            out.append({"mode": f"reader_{kind}", "samples": n, "snps": m, "file_MB": round(size / 1e6, 1),
+                        "seconds": round(dt, 4), "file_GB_per_s": round(size / dt / 1e9, 3),
+                        "genotypes_per_s": round(n * m / dt, 1), "matches_source": same})
+            del Gr
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--mode", choices=["projection", "supervised", "cv", "init", "all"], default="all")
+    ap.add_argument("--mode", choices=["projection", "supervised", "cv", "init", "readers", "all"], default="all")
     ap.add_argument("--workload", default="biobank", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="bf16", choices=["f64", "f32", "bf16"])
     ap.add_argument("--iters", type=int, default=20)
@@ -117,6 +169,10 @@ def main(argv=None) -> int:
     ap.add_argument("--timed-folds", type=int, default=1)
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
+    if args.mode == "readers":
+        for rec in time_readers(dev):
+            print(json.dumps(rec), flush=True)
+        return 0
     w = WORKLOADS[args.workload]
     G, P0, Q0 = make_genotypes(w, dev)
     base = {"workload": f"{w.name}: N={w.N} M={w.M} K={w.K}"}

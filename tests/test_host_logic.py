@@ -135,3 +135,15 @@ def test_pgen_layout_matches_oracle_header():
         pgen_layout(data[:2] + b"\x01" + data[3:])          # fixed-width storage mode
     with pytest.raises(ValueError):
         pgen_layout(data[:-1])                                # truncated records
+
+
+def test_timing_vcf_writer_round_trips_through_oracle_parser(tmp_path):
+    """The reader-throughput mode's synthetic VCF decodes (oracle parser) to
+    the codes it was written from, missing included."""
+    import oracle as O
+    from adamixture_amd.mode_timing import _write_vcf
+    G = np.random.default_rng(0).integers(0, 4, size=(1500, 37)).astype(np.uint8)
+    _write_vcf(tmp_path / "x.vcf", G)
+    got, N, M = O.vcf_to_unpacked((tmp_path / "x.vcf").read_bytes())
+    assert (N, M) == (37, 1500)
+    np.testing.assert_array_equal(got, G)
