@@ -20,7 +20,7 @@ ADX_F64 = 1
 ADX_BF16 = 2
 ADX_EM_STATS = 1
 ADX_EM_LOGLIK = 2
-ADX_CV_BLOCKS = 1024          # include/adx.h
+ADX_CV_BLOCKS = 4096          # include/adx.h
 ADX_GEMM_TN_BLOCKS = 512      # include/adx.h
 
 _i64 = ctypes.c_int64

@@ -15,6 +15,7 @@ import time
 import torch
 
 from . import _lib
+from .als_gpu import _nn, _tn
 from .em_adam_gpu import em_pass
 from .genotypes import PackedGenotypes, as_packed
 from .solvers import sqp_solve_p, sqp_solve_q, v_kk
@@ -109,13 +110,15 @@ def _optimize_original_gpu_once(G, P, Q, max_iter, K, M, N, tol, Q_hist, patienc
         V[:, col] = x2 - x1
         nc = min(it, Q_hist)
         Us, Vs = U[:, :nc], V[:, :nc]
-        lhs = Us.T @ (Us - Vs)
-        rhs = Us.T @ (x - x1)
+        # skinny products (dim x nc, nc <= Q_hist) in libadx: the library
+        # GEMMs take ~0.1 s per call at these shapes
+        lhs = _tn(Us, Us - Vs)
+        rhs = _tn(Us, (x - x1).unsqueeze(1))
         try:
             alpha = torch.linalg.solve(lhs, rhs)
         except RuntimeError:
             alpha = torch.linalg.lstsq(lhs, rhs).solution
-        x_qn = x1 - Vs @ alpha
+        x_qn = x1 - _nn(Vs, alpha).squeeze(1)
         P_qn = x_qn[:M * K].view(M, K).clone()
         Q_qn = x_qn[M * K:].view(N, K).clone()
         map_pq(P_qn, Q_qn)

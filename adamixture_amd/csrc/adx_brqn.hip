@@ -28,15 +28,18 @@ struct Weights {
     double z1, z2, w1, w2;
 };
 
+// Two FP64 divisions (1/H, 1/(1-H)) instead of four; each weight then
+// differs from the reference's quotient by at most 2 ulp.
 __device__ __forceinline__ Weights bq_weights(double h, uint32_t code) {
     Weights w{0.0, 0.0, 0.0, 0.0};
     if (code == 3u) return w;
     const double hc = fmax(fmin(h, 1.0 - 1e-10), 1e-10);
     const double g = (double)code;
-    w.z1 = g / hc;
-    w.z2 = (2.0 - g) / (1.0 - hc);
-    w.w1 = w.z1 / hc;
-    w.w2 = w.z2 / (1.0 - hc);
+    const double rh = 1.0 / hc, ru = 1.0 / (1.0 - hc);
+    w.z1 = g * rh;
+    w.z2 = (2.0 - g) * ru;
+    w.w1 = w.z1 * rh;
+    w.w2 = w.z2 * ru;
     return w;
 }
 
@@ -216,14 +219,17 @@ int adx_grad_hess(int side, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
     const int64_t rows = side == 0 ? N : M;
     ADX_CHECK(hipMemsetAsync(XtX, 0, sizeof(double) * (size_t)(rows * K * K), st));
     ADX_CHECK(hipMemsetAsync(Xtz, 0, sizeof(double) * (size_t)(rows * K), st));
-    if (K <= 4)
-        launch_gh<4>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
-    else if (K <= 8)
-        launch_gh<8>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
-    else if (K <= 12)
-        launch_gh<12>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
-    else
-        launch_gh<16>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz);
+    // register blocks padded to an even K (the K(K+1)/2 Hessian terms of
+    // each thread are the bulk of the work)
+    switch ((K + 1) / 2) {
+        case 1: case 2: launch_gh<4>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+        case 3: launch_gh<6>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+        case 4: launch_gh<8>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+        case 5: launch_gh<10>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+        case 6: launch_gh<12>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+        case 7: launch_gh<14>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+        default: launch_gh<16>(side, st, G, ldg, M, N, K, Q, P, XtX, Xtz); break;
+    }
     ADX_LAUNCH_CHECK("adx_grad_hess");
     return 0;
 }

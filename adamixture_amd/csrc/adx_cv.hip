@@ -37,23 +37,65 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     return z ^ (z >> 31);
 }
 
+// Fold test of the hashed path, splitmix64(idx + seed) mod n_folds == fold,
+// with the modulo by the run-time n_folds done as a multiply-high by a
+// precomputed magic (the division-by-invariant-integer method: q =
+// (hi(x m) + ((x - hi(x m)) >> 1)) >> (s - 1), exact for every 64-bit x;
+// a 64-bit % by a variable is a long instruction sequence on the GPU).
+struct FoldHash {
+    uint64_t seed, magic;
+    uint64_t d;
+    int shift, fold;
+    __device__ __forceinline__ bool held(uint64_t idx) const {
+        const uint64_t x = splitmix64(idx + seed);
+        uint64_t r = 0;
+        if (shift > 0) {
+            const uint64_t hi = __umul64hi(x, magic);
+            const uint64_t q = ((((x - hi) >> 1) + hi) >> (shift - 1));
+            r = x - q * d;
+        }
+        return r == (uint64_t)fold;
+    }
+};
+
+static FoldHash fold_hash(uint64_t seed, int n_folds, int fold) {
+    FoldHash h;
+    h.seed = seed;
+    h.d = (uint64_t)n_folds;
+    h.fold = fold;
+    if (n_folds == 1) {
+        h.shift = 0;
+        h.magic = 0;
+    } else {
+        h.shift = 64 - __builtin_clzll(h.d - 1);
+        const unsigned __int128 one = 1;
+        h.magic = (uint64_t)(((one << (64 + h.shift)) / h.d) - (one << 64) + 1);
+    }
+    return h;
+}
+
 template <typename I>
 __device__ __forceinline__ int64_t list_entry(const void* flat, int64_t e, int64_t first, int64_t stride) {
     return (int64_t) reinterpret_cast<const I*>(flat)[first + e * stride];
 }
 
-// br_qn.pyx:121-147: one held-out entry's deviance term.
-__device__ __forceinline__ double deviance_term(uint32_t code, const double* __restrict__ p,
-                                                const double* __restrict__ q, int K) {
+// br_qn.pyx:121-147: one held-out entry's deviance term, from the dot
+// product q . p (summed over k in ascending order).
+__device__ __forceinline__ double deviance_from_mu(uint32_t code, double mu) {
     const double g = (double)code;
-    double mu = 0.0;
-    for (int k = 0; k < K; ++k) mu += q[k] * p[k];
     mu = 2.0 * mu;
     mu = mu < 1e-10 ? 1e-10 : (mu > 2.0 - 1e-10 ? 2.0 - 1e-10 : mu);
     const double rem = 2.0 - g;
     const double ta = g > 0.0 ? g * log(g / mu) : 0.0;
     const double tb = rem > 0.0 ? rem * log(rem / (2.0 - mu)) : 0.0;
     return ta + tb;
+}
+
+__device__ __forceinline__ double deviance_term(uint32_t code, const double* __restrict__ p,
+                                                const double* __restrict__ q, int K) {
+    double mu = 0.0;
+    for (int k = 0; k < K; ++k) mu += q[k] * p[k];
+    return deviance_from_mu(code, mu);
 }
 
 // Block sum of v into part[blockIdx.x] (256 threads, fixed tree).
@@ -80,8 +122,7 @@ __global__ __launch_bounds__(256) void cv_mask_list_kernel(uint8_t* __restrict__
 }
 
 __global__ __launch_bounds__(256) void cv_mask_hash_kernel(const uint8_t* __restrict__ G, uint8_t* __restrict__ Gm,
-                                                           int64_t ldg, int64_t M, int64_t N, uint64_t seed,
-                                                           int n_folds, int fold) {
+                                                           int64_t ldg, int64_t M, int64_t N, FoldHash fh) {
     const int64_t Mb = (M + 3) >> 2;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < Mb * ldg;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -92,9 +133,7 @@ __global__ __launch_bounds__(256) void cv_mask_hash_kernel(const uint8_t* __rest
             for (int f = 0; f < 4; ++f) {
                 const int64_t m = 4 * j + f;
                 const uint32_t code = (b >> (2 * f)) & 3u;
-                if (m < M && code != 3u &&
-                    splitmix64((uint64_t)(m * N + i) + seed) % (uint64_t)n_folds == (uint64_t)fold)
-                    b |= 3u << (2 * f);
+                if (m < M && code != 3u && fh.held((uint64_t)(m * N + i))) b |= 3u << (2 * f);
             }
         }
         Gm[e] = (uint8_t)b;
@@ -120,30 +159,48 @@ __global__ __launch_bounds__(256) void cv_dev_list_kernel(const uint8_t* __restr
     block_sum_store(acc, part);
 }
 
-// Byte rows j = blockIdx.x, blockIdx.x + gridDim.x, ...; samples strided
-// over the threads (coalesced byte reads); the 4 P rows of a byte row come
-// from the cache, shared by every thread.
+// Thread = sample i (its Q row in registers when K <= KQ); workgroup = 256
+// consecutive samples (blocks strided by gridDim.x) x one range of byte rows
+// (blockIdx.y).  Each genotype byte is a coalesced read and the 4 P rows of
+// a byte row are the same address in every lane.
+constexpr int KQ = 16;
+
 __global__ __launch_bounds__(256) void cv_dev_hash_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
-                                                          int64_t N, uint64_t seed, int n_folds, int fold,
-                                                          const double* __restrict__ P,
+                                                          int64_t N, FoldHash fh, const double* __restrict__ P,
                                                           const double* __restrict__ Q, int K,
-                                                          double* __restrict__ part) {
+                                                          int64_t rows_per_split, double* __restrict__ part) {
     const int64_t Mb = (M + 3) >> 2;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
+    const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
+    const int64_t nsb = (N + 255) / 256;
     double acc = 0.0;
-    for (int64_t j = blockIdx.x; j < Mb; j += gridDim.x) {
-        for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-            const uint32_t b = G[j * ldg + i];
+    for (int64_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
+        const int64_t i = sb * 256 + threadIdx.x;
+        if (i >= N) continue;
+        double q[KQ];
+#pragma unroll
+        for (int k = 0; k < KQ; ++k) q[k] = k < K ? Q[i * K + k] : 0.0;
+        for (int64_t r = r0; r < r1; ++r) {
+            const uint32_t b = G[r * ldg + i];
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
-                const int64_t m = 4 * j + f;
+                const int64_t m = 4 * r + f;
                 const uint32_t code = (b >> (2 * f)) & 3u;
-                if (m < M && code != 3u &&
-                    splitmix64((uint64_t)(m * N + i) + seed) % (uint64_t)n_folds == (uint64_t)fold)
-                    acc += deviance_term(code, P + m * K, Q + i * K, K);
+                if (m < M && code != 3u && fh.held((uint64_t)(m * N + i))) {
+                    if (K <= KQ) {
+                        double mu = 0.0;
+#pragma unroll
+                        for (int k = 0; k < KQ; ++k)
+                            if (k < K) mu += q[k] * P[m * K + k];
+                        acc += deviance_from_mu(code, mu);
+                    } else {
+                        acc += deviance_term(code, P + m * K, Q + i * K, K);
+                    }
+                }
             }
         }
     }
-    block_sum_store(acc, part);
+    block_sum_store(acc, part + (int64_t)blockIdx.y * gridDim.x);
 }
 
 // out[0] = sum of part[0 .. n) in a fixed order (one workgroup).
@@ -194,7 +251,8 @@ int adx_cv_mask_hash(const uint8_t* G, uint8_t* Gm, int64_t ldg, int64_t M, int6
     if (n_folds < 1 || fold < 0 || fold >= n_folds) return fail_msg(1, "adx_cv_mask_hash: bad fold");
     int64_t blocks = ceil_div(ceil_div(M, 4) * ldg, 256);
     if (blocks > 65536) blocks = 65536;
-    cv_mask_hash_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(G, Gm, ldg, M, N, seed, n_folds, fold);
+    cv_mask_hash_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(G, Gm, ldg, M, N,
+                                                                        fold_hash(seed, n_folds, fold));
     ADX_LAUNCH_CHECK("adx_cv_mask_hash");
     return 0;
 }
@@ -226,9 +284,18 @@ int adx_cv_deviance_hash(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, ui
     if (K < 1 || n_folds < 1 || fold < 0 || fold >= n_folds) return fail_msg(1, "adx_cv_deviance_hash: bad K / fold");
     hipStream_t st = (hipStream_t)stream;
     const int64_t Mb = ceil_div(M, 4);
-    const int blocks = (int)(Mb < ADX_CV_BLOCKS ? Mb : ADX_CV_BLOCKS);
-    cv_dev_hash_kernel<<<blocks, 256, 0, st>>>(G, ldg, M, N, seed, n_folds, fold, P, Q, K, work);
-    cv_sum_kernel<<<1, 256, 0, st>>>(work, blocks, out);
+    // at most ADX_CV_BLOCKS workgroups (one partial each): sample blocks x
+    // byte-row splits
+    const int64_t nsb = ceil_div(N, 256);
+    const int64_t gx = nsb < ADX_CV_BLOCKS ? nsb : ADX_CV_BLOCKS;
+    int64_t gy = ADX_CV_BLOCKS / gx;
+    if (gy > Mb) gy = Mb;
+    if (gy < 1) gy = 1;
+    const int64_t rps = ceil_div(Mb, gy);
+    gy = ceil_div(Mb, rps);
+    dim3 grid((unsigned)gx, (unsigned)gy);
+    cv_dev_hash_kernel<<<grid, 256, 0, st>>>(G, ldg, M, N, fold_hash(seed, n_folds, fold), P, Q, K, rps, work);
+    cv_sum_kernel<<<1, 256, 0, st>>>(work, (int)(gx * gy), out);
     ADX_LAUNCH_CHECK("adx_cv_deviance_hash");
     return 0;
 }

@@ -25,6 +25,7 @@ import time
 
 import torch
 
+from .als_gpu import _nn, _tn
 from .br_qn_gpu import _loglik, grad_hess
 from .em_adam_gpu import EMAdamOptimizer, em_pass
 from .genotypes import as_packed
@@ -136,13 +137,15 @@ def optimize_projection_original_gpu(G, P, Q, max_iter, K, M, N, tol, Q_hist, pa
         V[:, col] = x2 - x1
         nc = min(it, Q_hist)
         Us, Vs = U[:, :nc], V[:, :nc]
-        lhs = Us.T @ (Us - Vs)
-        rhs = Us.T @ (x - x1)
+        # skinny products (dim x nc, nc <= Q_hist) in libadx: the library
+        # GEMMs take ~0.1 s per call at these shapes
+        lhs = _tn(Us, Us - Vs)
+        rhs = _tn(Us, (x - x1).unsqueeze(1))
         try:
             alpha = torch.linalg.solve(lhs, rhs)
         except RuntimeError:
             alpha = torch.linalg.lstsq(lhs, rhs).solution
-        Q_qn = (x1 - Vs @ alpha).view(N, K).clone()
+        Q_qn = (x1 - _nn(Vs, alpha).squeeze(1)).view(N, K).clone()
         torch.clamp_(Q_qn, 1e-5, 1.0 - 1e-5)
         Q_qn.div_(Q_qn.sum(dim=1, keepdim=True))
         ll_qn = _loglik(G, P, Q_qn)

@@ -377,7 +377,7 @@ int adx_clear_tail_fields(uint8_t *G, int64_t ldg, int64_t M, int64_t N, void *s
 /* --- cross-validation ------------------------------------------------------ */
 
 /* Partial-sum slots the deviance entry points need in `work` (doubles). */
-#define ADX_CV_BLOCKS 1024
+#define ADX_CV_BLOCKS 4096
 
 /* A fold given as a list of flat indices f = m N + i (int32 if idx64 == 0,
  * else int64), entries flat[first + e * stride] for e < n (the reference's
