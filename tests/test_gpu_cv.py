@@ -134,3 +134,48 @@ def test_cv_index_synthetic_hashed(cuda):
     want = O.cross_validation(O.unpack(G.to_host_packed(), w.M), P, Q, 123, 3, streaming=True)
     print(f"CV index gpu {got:.12f} oracle {want:.12f}")
     assert abs(got - want) <= 1e-9 * abs(want)
+
+
+@pytest.mark.parametrize("n_folds", [1, 2, 3, 7, 10, 64, 1000])
+def test_hash_masks_for_many_fold_counts(cuda, fixtures, n_folds):
+    """The hashed fold test (a 64-bit modulo by the run-time fold count,
+    done with a precomputed multiply-high magic) against the oracle's
+    Python modulo, for fold counts of every kind (1, powers of two, odd,
+    larger than the data's byte values)."""
+    G = fixtures["reader_G"]
+    M, N = G.shape
+    Gp = as_packed(fixtures["reader_G_packed"], M, cuda)
+    Gm = PackedGenotypes(torch.empty_like(Gp.data), M, N)
+    obs = np.flatnonzero(G != 3)
+    folds = O.cv_hash_folds(obs, 12345, n_folds)
+    for fold in sorted({0, n_folds // 2, n_folds - 1}):
+        print(f"case: hash mask n_folds={n_folds} fold={fold}", flush=True)
+        _lib.call("adx_cv_mask_hash", Gp.data.data_ptr(), Gm.data.data_ptr(), Gp.ldg, M, N, 12345, n_folds, fold,
+                  _lib.stream_handle(cuda))
+        want = G.copy()
+        want.ravel()[obs[folds == fold]] = 3
+        np.testing.assert_array_equal(_unpack(Gm), want)
+
+
+@pytest.mark.parametrize("K", [3, 16, 20])
+def test_hash_deviance_register_and_pointer_paths(cuda, K):
+    """Hashed deviance with the Q row held in registers (K <= 16) and read
+    from memory (K > 16), on a synthetic matrix whose N is not a multiple of
+    the 256-sample thread block, against the oracle."""
+    from adamixture_amd.synth import Workload, make_genotypes
+    w = Workload("cvdev", 333, 2050, K, "bn", 0.3, 0.05, fst=0.1, seed=K)
+    G, P0, Q0 = make_genotypes(w, cuda)
+    P = np.clip(P0, 1e-5, 1 - 1e-5)
+    Q = Q0 / Q0.sum(axis=1, keepdims=True)
+    Gu = O.unpack(G.to_host_packed(), w.M)
+    obs = np.flatnonzero(Gu != 3)
+    work = torch.empty(_lib.ADX_CV_BLOCKS, dtype=torch.float64, device=cuda)
+    out = torch.empty(1, dtype=torch.float64, device=cuda)
+    Pd, Qd = torch.from_numpy(P).to(cuda), torch.from_numpy(Q).to(cuda)
+    for fold in (0, 2):
+        print(f"case: hash deviance K={K} fold={fold}", flush=True)
+        held = obs[O.cv_hash_folds(obs, 77, 3) == fold]
+        want = O.cv_deviance_sum(Gu.ravel()[held], held, P, Q, w.N)
+        _lib.call("adx_cv_deviance_hash", G.data.data_ptr(), G.ldg, w.M, w.N, 77, 3, fold, Pd.data_ptr(),
+                  Qd.data_ptr(), K, work.data_ptr(), out.data_ptr(), _lib.stream_handle(cuda))
+        assert abs(out.item() - want) <= 1e-12 * abs(want)
