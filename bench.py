@@ -173,6 +173,19 @@ def main():
         traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
     achieved = flops / (kern_ms * 1e-3) / 1e12
     peak = PEAK_MFMA_TFLOPS[args.precision]
+    # The pass is bound by VALU issue, not by MFMA or HBM: its vector
+    # instruction count per dispatch (SQ_INSTS_VALU - SQ_INSTS_MFMA, from the
+    # committed counter run of this workload and build) against the issue
+    # capacity of the measured kernel time (1024 SIMDs x 2.4 GHz, one wave64
+    # VALU instruction per 4 cycles).
+    valu = None
+    sqfile = ROOT / "profiles" / f"em_pass_{args.precision}_{w.name}_sq_counters.json"
+    if sqfile.exists():
+        sq = json.loads(sqfile.read_text())
+        n_valu = sq["SQ_INSTS_VALU"] - sq["SQ_INSTS_MFMA"]
+        cap = 1024 * 2.4e9 * kern_ms * 1e-3 / 4.0
+        valu = {"instructions": n_valu, "issue_capacity": round(cap), "frac": round(n_valu / cap, 4),
+                "mfma_busy_frac": round(sq["SQ_VALU_MFMA_BUSY_CYCLES"] / (1024 * 2.4e9 * kern_ms * 1e-3), 4)}
     line = {
         "metric": "EM iters/s (Adam-EM, N=100k M=500k K=10)",
         "value": round(args.steps / elapsed, 4),
@@ -195,7 +208,8 @@ def main():
                      "kernel": "em_pass_bf16_kernel" if args.precision == "bf16" else "em_pass_kernel",
                      "kernel_ms": round(kern_ms, 4),
                      "algorithmic_flops": flops, "algorithmic_bytes": bytes_alg,
-                     "hbm_GBs_algorithmic": round(bytes_alg / (kern_ms * 1e-3) / 1e9, 1)},
+                     "hbm_GBs_algorithmic": round(bytes_alg / (kern_ms * 1e-3) / 1e9, 1),
+                     "valu_issue": valu},
     }
     from adamixture_amd.autotune import get_tiles
     line["config"]["tiles"] = dict(zip(("cw", "target_wgs"), get_tiles(args.precision)))
