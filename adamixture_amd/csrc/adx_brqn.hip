@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
     const int64_t Mb = (M + 3) >> 2;
     const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
     const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
+    const int64_t m_end = 4 * r1 < M ? 4 * r1 : M;     // SNPs of this split: 4 r0 .. m_end - 1
     const bool live = i < N;
     double q[KM], hz[KM], hx[KT];
 #pragma unroll
@@ -62,34 +63,52 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
     }
 #pragma unroll
     for (int t = 0; t < KT; ++t) hx[t] = 0.0;
+    if (4 * r0 >= m_end) return;
+    // The P row of the next SNP (the same address in every lane) and the
+    // genotype byte of the next row are loaded one step ahead.
+    double pc[KM];
+#pragma unroll
+    for (int k = 0; k < KM; ++k) pc[k] = k < K ? P[4 * r0 * K + k] : 0.0;
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    uint32_t bn = live ? G[r0 * ldg + i] : 0xFFu;
     for (int64_t r = r0; r < r1; ++r) {
-        const uint32_t byte = live ? G[r * ldg + i] : 0xFFu;
+        const uint32_t byte = bn;
+        if (r + 1 < r1) bn = live ? G[(r + 1) * ldg + i] : 0xFFu;
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             const int64_t m = 4 * r + f;
             if (m >= M) break;
+            const int64_t mn = m + 1 < m_end ? m + 1 : m;
+            // This row's scalar loads (issued one SNP ago) are complete
+            // before the next row's are issued: scalar loads return out of
+            // order, so a wait placed at the first use would also wait for
+            // the prefetch.  lgkmcnt(0), other counters left at their maximum.
+            __builtin_amdgcn_s_waitcnt(0xC07F);
+            double pn[KM];
+#pragma unroll
+            for (int k = 0; k < KM; ++k) pn[k] = P[mn * K + (k < K ? k : 0)];   // masked on use
+            // keep the scheduler from sinking those loads to their use
+            __builtin_amdgcn_sched_barrier(0);
             const uint32_t code = (byte >> (2 * f)) & 3u;
-            double p[KM];
             double h = 0.0;
 #pragma unroll
-            for (int k = 0; k < KM; ++k) {
-                p[k] = k < K ? P[m * K + k] : 0.0;
-                h += q[k] * p[k];
-            }
+            for (int k = 0; k < KM; ++k) h += q[k] * pc[k];
             const Weights w = bq_weights(h, code);
             double a[KM], b[KM];
 #pragma unroll
             for (int k = 0; k < KM; ++k) {
-                const double u = 1.0 - p[k];
-                hz[k] += w.z1 * p[k] + w.z2 * u;
-                a[k] = w.w1 * p[k];
+                const double u = 1.0 - pc[k];
+                hz[k] += w.z1 * pc[k] + w.z2 * u;
+                a[k] = w.w1 * pc[k];
                 b[k] = w.w2 * u;
             }
             int t = 0;
 #pragma unroll
             for (int k = 0; k < KM; ++k)
 #pragma unroll
-                for (int l = k; l < KM; ++l, ++t) hx[t] += a[k] * p[l] + b[k] * (1.0 - p[l]);
+                for (int l = k; l < KM; ++l, ++t) hx[t] += a[k] * pc[l] + b[k] * (1.0 - pc[l]);
+#pragma unroll
+            for (int k = 0; k < KM; ++k) pc[k] = k < K ? pn[k] : 0.0;
         }
     }
     if (!live) return;
