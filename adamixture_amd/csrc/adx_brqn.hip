@@ -13,7 +13,7 @@
 // streams the 2-bit genotypes once:
 //   Q side: thread = sample, workgroup = 256 consecutive samples x a SNP
 //           range; one coalesced byte per 4 SNPs, P rows are wave-uniform.
-//   P side: thread = SNP, workgroup = 64 SNPs; 256-sample chunks of the
+//   P side: thread = SNP, workgroup = 256 SNPs; 256-sample chunks of the
 //           genotype rows and of Q are staged in LDS (Q reads broadcast).
 // Sample splits / SNP splits are combined with FP64 atomics.
 #include <hip/hip_runtime.h>
@@ -126,17 +126,21 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
 }
 
 template <int KM>
-__global__ __launch_bounds__(64) void grad_hess_p_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
-                                                         int64_t N, int K, const double* __restrict__ Q,
-                                                         const double* __restrict__ P, int64_t n_per_split,
-                                                         double* __restrict__ XtX, double* __restrict__ Xtz) {
+__global__ __launch_bounds__(256) void grad_hess_p_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                          int64_t N, int K, const double* __restrict__ Q,
+                                                          const double* __restrict__ P, int64_t n_per_split,
+                                                          double* __restrict__ XtX, double* __restrict__ Xtz) {
     constexpr int KT = KM * (KM + 1) / 2;
-    constexpr int CH = 256;  // samples per LDS chunk
-    __shared__ uint8_t sg[16][CH];
-    __shared__ double sq[CH][KM];
+    constexpr int CH = 256;   // samples per LDS chunk
+    constexpr int SGS = CH + 4;  // byte-row stride: the 16 rows a wave reads fall on distinct banks
+    // Workgroup = 256 SNPs = 64 byte rows; thread = SNP (byte row tid / 4,
+    // field tid % 4).  Each chunk stages the 64 rows' genotype bytes and the
+    // chunk's Q rows (read by every lane alike: LDS broadcasts).
+    __shared__ uint8_t sg[64 * SGS];
+    __shared__ __attribute__((aligned(16))) double sq[CH * KM];
     const int tid = threadIdx.x;
-    const int64_t m = (int64_t)blockIdx.x * 64 + tid;
-    const int64_t row0 = (int64_t)blockIdx.x * 16;
+    const int64_t m = (int64_t)blockIdx.x * 256 + tid;
+    const int64_t row0 = (int64_t)blockIdx.x * 64;
     const int64_t Mb = (M + 3) >> 2;
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
@@ -149,39 +153,50 @@ __global__ __launch_bounds__(64) void grad_hess_p_kernel(const uint8_t* __restri
     }
 #pragma unroll
     for (int t = 0; t < KT; ++t) hx[t] = 0.0;
+    const int rr = tid >> 2, sh = 2 * (tid & 3);
     for (int64_t c0 = n0; c0 < n1; c0 += CH) {
         const int nc = (int)(n1 - c0 < CH ? n1 - c0 : CH);
         __syncthreads();
-        for (int e = tid; e < 16 * CH; e += 64) {
-            const int r = e / CH, c = e - r * CH;
-            sg[r][c] = (row0 + r < Mb && c < nc) ? G[(row0 + r) * ldg + c0 + c] : 0xFFu;
+        // 64 rows x CH bytes: thread -> row e / 16, 16-byte column e % 16
+        for (int e = tid; e < 64 * (CH / 16); e += 256) {
+            const int r = e / (CH / 16), c = 16 * (e - r * (CH / 16));
+            uint4 v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+            if (row0 + r < Mb && c < nc) v = *reinterpret_cast<const uint4*>(G + (row0 + r) * ldg + c0 + c);
+            uint32_t* d = reinterpret_cast<uint32_t*>(sg + r * SGS + c);
+            d[0] = v.x;
+            d[1] = v.y;
+            d[2] = v.z;
+            d[3] = v.w;
         }
-        for (int e = tid; e < CH * KM; e += 64) {
+        for (int e = tid; e < CH * KM; e += 256) {
             const int c = e / KM, k = e - c * KM;
-            sq[c][k] = (c < nc && k < K) ? Q[(c0 + c) * K + k] : 0.0;
+            sq[e] = (c < nc && k < K) ? Q[(c0 + c) * K + k] : 0.0;
         }
         __syncthreads();
         if (!live) continue;
-        const int rr = tid >> 2, sh = 2 * (tid & 3);
         for (int c = 0; c < nc; ++c) {
-            const uint32_t code = (sg[rr][c] >> sh) & 3u;
+            const uint32_t code = (sg[rr * SGS + c] >> sh) & 3u;
+            const double* qc = sq + c * KM;
+            double qv[KM];
             double h = 0.0;
 #pragma unroll
-            for (int k = 0; k < KM; ++k) h += sq[c][k] * p[k];
+            for (int k = 0; k < KM; ++k) {
+                qv[k] = qc[k];
+                h += qv[k] * p[k];
+            }
             const Weights w = bq_weights(h, code);
             const double zd = w.z1 - w.z2, ws = w.w1 + w.w2;
             double wq[KM];
 #pragma unroll
             for (int k = 0; k < KM; ++k) {
-                const double qk = sq[c][k];
-                hz[k] += zd * qk;
-                wq[k] = ws * qk;
+                hz[k] += zd * qv[k];
+                wq[k] = ws * qv[k];
             }
             int t = 0;
 #pragma unroll
             for (int k = 0; k < KM; ++k)
 #pragma unroll
-                for (int l = k; l < KM; ++l, ++t) hx[t] += wq[k] * sq[c][l];
+                for (int l = k; l < KM; ++l, ++t) hx[t] += wq[k] * qv[l];
         }
     }
     if (!live) return;
@@ -211,14 +226,14 @@ static void launch_gh(int side, hipStream_t st, const uint8_t* G, int64_t ldg, i
         dim3 grid((unsigned)nbx, (unsigned)ceil_div(Mb, rps));
         grad_hess_q_kernel<KM><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, rps, XtX, Xtz);
     } else {
-        const int64_t nbx = ceil_div(M, 64);
-        int64_t s = ceil_div(8192, nbx);
+        const int64_t nbx = ceil_div(M, 256);
+        int64_t s = ceil_div(2048, nbx);
         const int64_t max_s = ceil_div(N, 256);
         s = s > max_s ? max_s : (s < 1 ? 1 : s);
         if (s > 65535) s = 65535;
         const int64_t nps = ceil_div(ceil_div(N, s), 256) * 256;
         dim3 grid((unsigned)nbx, (unsigned)ceil_div(N, nps));
-        grad_hess_p_kernel<KM><<<grid, 64, 0, st>>>(G, ldg, M, N, K, Q, P, nps, XtX, Xtz);
+        grad_hess_p_kernel<KM><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, nps, XtX, Xtz);
     }
 }
 
@@ -233,7 +248,8 @@ int adx_grad_hess(int side, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
     if (M <= 0 || N <= 0) return 0;
     if (K < 1 || K > 16) return fail_msg(1, "adx_grad_hess: K must be in [1, 16]");
     if (side != 0 && side != 1) return fail_msg(1, "adx_grad_hess: side must be 0 (Q) or 1 (P)");
-    if (ldg < N) return fail_msg(1, "adx_grad_hess: ldg < N");
+    if (ldg < N || ldg % 16 != 0) return fail_msg(1, "adx_grad_hess: ldg must be >= N and a multiple of 16");
+    if (reinterpret_cast<uintptr_t>(G) % 16 != 0) return fail_msg(1, "adx_grad_hess: G not 16-byte aligned");
     hipStream_t st = (hipStream_t)stream;
     const int64_t rows = side == 0 ? N : M;
     ADX_CHECK(hipMemsetAsync(XtX, 0, sizeof(double) * (size_t)(rows * K * K), st));

@@ -217,7 +217,8 @@ int64_t adx_svd_work_floats(int which, int64_t M, int64_t N, int L);
  *   side 1: Xtz (M x K)  = sum_i (g/H - (2-g)/(1-H)) q_i
  *           XtX (M x K x K) = sum_i (g/H^2 + (2-g)/(1-H)^2) q_i q_i^T
  * (positive gradient convention; the reference negates before the SQP
- * solve).  Outputs are overwritten. */
+ * solve).  Outputs are overwritten.  G 16-byte aligned, ldg >= N and a
+ * multiple of 16 (the padded device layout). */
 int adx_grad_hess(int side, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, const double *Q,
                   const double *P, double *XtX, double *Xtz, void *stream);
 
