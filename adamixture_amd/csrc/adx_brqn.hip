@@ -28,14 +28,25 @@ struct Weights {
     double z1, z2, w1, w2;
 };
 
-// Two FP64 divisions (1/H, 1/(1-H)) instead of four; each weight then
-// differs from the reference's quotient by at most 2 ulp.
+// 1/x for x in [1e-10, 1]: the hardware reciprocal estimate refined by
+// two Newton-Raphson steps r <- r + r (1 - x r) (the error squares at each
+// step), within a few ulp of the correctly rounded quotient.
+__device__ __forceinline__ double recip_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Two reciprocals (1/H, 1/(1-H)) instead of the reference's four
+// quotients; each weight then differs from its quotient by a few ulp.
 __device__ __forceinline__ Weights bq_weights(double h, uint32_t code) {
     Weights w{0.0, 0.0, 0.0, 0.0};
     if (code == 3u) return w;
     const double hc = fmax(fmin(h, 1.0 - 1e-10), 1e-10);
     const double g = (double)code;
-    const double rh = 1.0 / hc, ru = 1.0 / (1.0 - hc);
+    const double rh = recip_nr(hc), ru = recip_nr(1.0 - hc);
     w.z1 = g * rh;
     w.z2 = (2.0 - g) * ru;
     w.w1 = w.z1 * rh;
