@@ -46,13 +46,26 @@ __device__ __forceinline__ void em_terms<float>(float h, uint32_t code, float& a
     b = cb * (hc * r);
 }
 
-// FP64: the reference's divisions (em_adam_gpu.py:62-67).
+// FP64: the reference's quotients g / H and (2-g) / (1-H)
+// (em_adam_gpu.py:62-67) as g and 2-g times 1/H and 1/(1-H), each
+// reciprocal the hardware estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r) (x in [1e-5, 1]); a and b stay within a few ulp of
+// the correctly rounded quotients, the size of a change of summation order
+// in the contractions that follow.
+__device__ __forceinline__ double recip_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 template <>
 __device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double& a, double& b) {
     const double hc = fmin(fmax(h, 1e-5), 1.0 - 1e-5);
     const double g = (double)code;
-    a = code == 3u ? 0.0 : g / hc;
-    b = code == 3u ? 0.0 : (2.0 - g) / (1.0 - hc);
+    a = code == 3u ? 0.0 : g * recip_nr(hc);
+    b = code == 3u ? 0.0 : (2.0 - g) * recip_nr(1.0 - hc);
 }
 
 // Log-likelihood of em_adam_gpu.py:238-246 (H clamped to [1e-10, 1-1e-10],
