@@ -173,7 +173,12 @@ __global__ __launch_bounds__(256) void cv_dev_hash_kernel(const uint8_t* __restr
     const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
     const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
     const int64_t nsb = (N + 255) / 256;
-    double acc = 0.0;
+    // Per held-out entry, g log(g/mu) + (2-g) log((2-g)/(2-mu)) =
+    //   [g log g + (2-g) log(2-g)] - log(mu^g (2-mu)^(2-g)):
+    // the first part is 2 ln 2 for g = 0 or 2 and 0 for g = 1 (counted), the
+    // second a log-product (factors >= 1e-20, one log per thread at the end).
+    LogProd lp;
+    int64_t n_hom = 0;
     for (int64_t sb = blockIdx.x; sb < nsb; sb += gridDim.x) {
         const int64_t i = sb * 256 + threadIdx.x;
         if (i >= N) continue;
@@ -187,19 +192,26 @@ __global__ __launch_bounds__(256) void cv_dev_hash_kernel(const uint8_t* __restr
                 const int64_t m = 4 * r + f;
                 const uint32_t code = (b >> (2 * f)) & 3u;
                 if (m < M && code != 3u && fh.held((uint64_t)(m * N + i))) {
+                    double mu = 0.0;
                     if (K <= KQ) {
-                        double mu = 0.0;
 #pragma unroll
                         for (int k = 0; k < KQ; ++k)
                             if (k < K) mu += q[k] * P[m * K + k];
-                        acc += deviance_from_mu(code, mu);
                     } else {
-                        acc += deviance_term(code, P + m * K, Q + i * K, K);
+                        for (int k = 0; k < K; ++k) mu += Q[i * K + k] * P[m * K + k];
                     }
+                    mu = 2.0 * mu;
+                    mu = mu < 1e-10 ? 1e-10 : (mu > 2.0 - 1e-10 ? 2.0 - 1e-10 : mu);
+                    const double x = code == 2u ? mu : 2.0 - mu, y = code == 0u ? 2.0 - mu : mu;
+                    lp.mul_factor(x * y);
+                    n_hom += code != 1u;
                 }
             }
+            lp.renorm();
         }
     }
+    const double ln2_hi = 0.6931471824645996, ln2_lo = -1.904654299957768e-09;
+    const double acc = 2.0 * ((double)n_hom * ln2_hi + (double)n_hom * ln2_lo) - lp.log_value();
     block_sum_store(acc, part + (int64_t)blockIdx.y * gridDim.x);
 }
 

@@ -68,37 +68,6 @@ __device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double
     b = code == 3u ? 0.0 : (2.0 - g) * recip_nr(1.0 - hc);
 }
 
-// Log-likelihood of em_adam_gpu.py:238-246 (H clamped to [1e-10, 1-1e-10],
-// g log H + (2-g) log(1-H) over observed genotypes) evaluated without a log
-// per genotype: each lane multiplies the per-genotype factors
-//   g = 0: (1-H)^2,  g = 1: H (1-H),  g = 2: H^2,  missing: 1
-// into an FP64 mantissa in [1, 2) and an integer binary exponent,
-// renormalised after every 4 factors (each >= 1e-20, so no underflow), and
-// takes a single log at the end.  The result differs from the sum of FP64
-// logs only by the product's roundings (<= 1.1e-16 each, random-walk
-// accumulated), which is below the FP64 summation error of that sum.
-struct LogProd {
-    double m = 1.0;
-    int64_t e = 0;
-    __device__ __forceinline__ void mul(double h, uint32_t code) {
-        const double hc = fmin(fmax(h, 1e-10), 1.0 - 1e-10);
-        const double u = 1.0 - hc;
-        const double f1 = code == 2u ? hc : u;
-        const double f2 = code == 0u ? u : hc;
-        m *= code == 3u ? 1.0 : f1 * f2;
-    }
-    __device__ __forceinline__ void renorm() {
-        const uint64_t bits = (uint64_t)__double_as_longlong(m);
-        e += (int64_t)((bits >> 52) & 0x7FF) - 1023;
-        m = __longlong_as_double((long long)((bits & 0x800FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL));
-    }
-    __device__ __forceinline__ double log_value() const {
-        // e * ln2 with ln2 split so that e * ln2_hi is exact for |e| < 2^29.
-        const double ln2_hi = 0.6931471824645996, ln2_lo = -1.904654299957768e-09;
-        return (double)e * ln2_hi + ((double)e * ln2_lo + log(m));
-    }
-};
-
 template <typename T, int KS, int CW, bool STATS, bool LL>
 __global__ __launch_bounds__(256) void em_pass_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
