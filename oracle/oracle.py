@@ -560,6 +560,14 @@ def map_q(Q):
     return Q
 
 
+def map_pq_gpu(P: np.ndarray, Q: np.ndarray):
+    """br_qn_gpu.py:47-61 _mapPQ_gpu: P and Q clipped to [1e-5, 1 - 1e-5],
+    then the rows of Q divided by their sums.  Returns new arrays."""
+    P = np.clip(P, 1e-5, 1 - 1e-5)
+    Q = np.clip(Q, 1e-5, 1 - 1e-5)
+    return P, Q / Q.sum(axis=1, keepdims=True)
+
+
 def als(U, S, V, f, seed, M, N, K, max_iter, tol, P_init=None, gpu_semantics=False):
     """als.py:10-88 ALS (CPU), or als_gpu.py:45-127 when ``gpu_semantics``.
 
@@ -706,8 +714,7 @@ def brqn(Gp, P, Q, M, *, max_iter=10000, tol=0.1, Q_hist=3, patience=3, vkk=None
         Pq = np.clip(xq[:M * K].reshape(M, K), 1e-5, 1 - 1e-5)
         Qq = xq[M * K:].reshape(N, K).copy()
         if gpu_semantics:
-            Qq = np.clip(Qq, 1e-5, 1 - 1e-5)
-            Qq /= Qq.sum(axis=1, keepdims=True)
+            Pq, Qq = map_pq_gpu(Pq, Qq)
         else:
             for i in range(N):
                 Qq[i] = project_simplex(Qq[i])

@@ -751,3 +751,20 @@ def test_learned_features_host_logic():
     np.testing.assert_allclose(F[:, 50], K / 16)
     np.testing.assert_allclose(F[:, 51], 1 / K)
     np.testing.assert_allclose(F[0, 52:52 + r - 1], (S[1:r] / S[0]).astype(np.float32), rtol=1e-5)
+
+
+def test_map_pq_gpu_pins_the_gpu_quasi_newton_map():
+    """br_qn_gpu.py:47-61 (the one step where the GPU-semantics BR-QN driver
+    differs from the pinned CPU driver): clip both to [1e-5, 1 - 1e-5], then
+    divide Q rows by their sums.  Feasible points are fixed points; the
+    bounds are the reference's; clipping comes before the division."""
+    P = np.array([[1e-5, 0.5, 1 - 1e-5], [-1.0, 0.0, 2.0]])
+    Q = np.array([[0.5, 0.25, 0.25], [3.0, 0.5, -1.0], [2.0, 2.0, 2.0]])
+    Pm, Qm = O.map_pq_gpu(P, Q)
+    np.testing.assert_array_equal(Pm, [[1e-5, 0.5, 1 - 1e-5], [1e-5, 1e-5, 1 - 1e-5]])
+    np.testing.assert_array_equal(Qm[0], Q[0])                        # on the simplex already
+    want = np.array([1 - 1e-5, 0.5, 1e-5]) / (1 - 1e-5 + 0.5 + 1e-5)  # clip, then divide
+    np.testing.assert_allclose(Qm[1], want, rtol=1e-15)
+    np.testing.assert_allclose(Qm[2], [1 / 3] * 3, rtol=1e-15)
+    np.testing.assert_allclose(Qm.sum(axis=1), 1.0, rtol=1e-15)
+    assert np.all(P == np.array([[1e-5, 0.5, 1 - 1e-5], [-1.0, 0.0, 2.0]]))  # inputs untouched
