@@ -2,6 +2,7 @@
 bvls_kernel torch ops, adamixture/src/utils_c/cuda/{sqp,bvls}_kernel.cu)."""
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -66,7 +67,7 @@ def batch_bvls_bpp(A: torch.Tensor, B: torch.Tensor, lower: float, upper: float,
         raise RuntimeError("A must be K x K")
     A = _f64c(A, dev)
     B = _f64c(B, dev)
-    Ainv = torch.linalg.inv(A).contiguous() if start == "inverse" else None
+    Ainv = _inv_host(A) if start == "inverse" else None
     X = torch.empty((M, K), dtype=torch.float64, device=dev)
     if Ainv is not None:
         nb = _lib.load().adx_bvls_workspace_bytes(K)
@@ -78,6 +79,17 @@ def batch_bvls_bpp(A: torch.Tensor, B: torch.Tensor, lower: float, upper: float,
     _lib.call("adx_bvls_batch", A.data_ptr(), None, B.data_ptr(), X.data_ptr(), M, K, lower, upper,
               q.data_ptr(), 8, _lib.stream_handle(dev))
     return X
+
+
+def _inv_host(A: torch.Tensor) -> torch.Tensor:
+    """inv(A) of the K x K matrix by LU with partial pivoting on the host (the
+    matrix is tiny; a device LU would load the solver library on first use).
+    A singular matrix raises RuntimeError, as torch.linalg.inv does."""
+    try:
+        inv = np.linalg.inv(A.cpu().numpy())
+    except np.linalg.LinAlgError as exc:
+        raise RuntimeError(f"batch_bvls_bpp: A is singular ({exc})") from exc
+    return torch.from_numpy(inv).to(A.device).contiguous()
 
 
 _BVLS_WS: dict = {}
