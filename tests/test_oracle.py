@@ -768,3 +768,18 @@ def test_map_pq_gpu_pins_the_gpu_quasi_newton_map():
     np.testing.assert_allclose(Qm[2], [1 / 3] * 3, rtol=1e-15)
     np.testing.assert_allclose(Qm.sum(axis=1), 1.0, rtol=1e-15)
     assert np.all(P == np.array([[1e-5, 0.5, 1 - 1e-5], [-1.0, 0.0, 2.0]]))  # inputs untouched
+
+
+def test_fast_oracle_build_leaves_the_fp_mode_alone():
+    """Loading the fast-math CPU-baseline build must not turn on
+    flush-to-zero / denormals-are-zero for the process (a -ffast-math link
+    of a shared library would): bench.py loads it before the
+    time-to-target race, whose host-side eigensolves and pseudo-inverses
+    would then differ."""
+    import subprocess
+    import sys
+    code = ("import numpy as np, oracle; oracle.lib_fast(); "
+            "print(float(np.float32(1e-40) * np.float32(1)), float(np.float64(1e-310) * 1.0))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True,
+                         cwd=str(__import__("pathlib").Path(__file__).resolve().parents[1])).stdout.split()
+    assert float(out[-2]) > 0.0 and float(out[-1]) > 0.0
