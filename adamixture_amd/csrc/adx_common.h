@@ -113,7 +113,7 @@ __device__ inline void project_simplex_row(const double* x, double* out, double*
 // per genotype: each lane multiplies the per-genotype factors
 //   g = 0: (1-H)^2,  g = 1: H (1-H),  g = 2: H^2,  missing: 1
 // into an FP64 mantissa in [1, 2) and an integer binary exponent,
-// renormalised after every 4 factors (each >= 1e-20, so no underflow), and
+// renormalised after at most 8 factors (each >= 1e-20, so no underflow), and
 // takes a single log at the end.  The result differs from the sum of FP64
 // logs only by the product's roundings (<= 1.1e-16 each, random-walk
 // accumulated), which is below the FP64 summation error of that sum.

@@ -192,7 +192,11 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                     if (STATS) em_terms<T>(h[j], code, a[j], b[j]);
                     if (LL) lp.mul((double)h[j], code);
                 }
-                if (LL) lp.renorm();
+                // 8 factors (two column blocks) between renormalisations: the
+                // mantissa stays >= 1e-160; the rescale is exact, so the result
+                // does not depend on where it happens.  CW is a multiple of 4,
+                // so the last column block of every row block renormalises.
+                if (LL && (cb & 1)) lp.renorm();
                 if (STATS) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {

@@ -93,7 +93,8 @@ int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int
  * reference): for dtype tag ADX_F32 / ADX_F64 (adx_em_pass) or ADX_BF16
  * (adx_em_pass_bf16), cw = 16-SNP column tiles per wavefront (4 or 8; 4 for
  * ADX_F64) and target_wgs = workgroups the sample range is split to reach
- * (64..65536).  Process-wide; affects later launches and
+ * (64..65536).  Process-wide and not synchronised between host threads
+ * (set it before passes run on other threads); affects later launches and
  * adx_em_workspace_bytes (query the workspace after changing it).  Defaults:
  * F32 (8, 8192), F64 (4, 4096), BF16 (8, 8192). */
 int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs);
