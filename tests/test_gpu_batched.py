@@ -129,3 +129,27 @@ def test_autotune_picks_a_candidate_and_keeps_results(cuda, precision):
         autotune.set_tiles(precision, *default)
     with pytest.raises(RuntimeError):
         autotune.set_tiles(precision, 5, 2048)
+
+
+def test_loglik_independent_of_tile_shape(cuda):
+    """The logL-only FP64 pass over every FP64 tile candidate (grids of
+    64..65536 workgroups change only the summation order of the per-lane
+    log-products): equal to 1e-13 relative, and the oracle's logL
+    (em_adam_gpu.py:238-246) to 1e-12."""
+    from adamixture_amd.em_adam_gpu import em_pass
+    G = _data(cuda, N=1500, M=7001, K=6, seed=16)
+    rng = np.random.default_rng(6)
+    P = torch.from_numpy(np.clip(rng.random((G.M, 6)), 1e-5, 1 - 1e-5)).to(cuda)
+    Q = torch.from_numpy(rng.dirichlet(np.ones(6), size=G.N)).to(cuda)
+    want = O.loglik_gpu(G.data[:, :G.N].cpu().numpy(), P.cpu().numpy(), Q.cpu().numpy(), G.M)
+    default = autotune.get_tiles("f64")
+    vals = []
+    try:
+        for cw, tg in autotune.CANDIDATES["f64"]:
+            print(f"case: logL tiles {cw}/{tg}", flush=True)
+            autotune.set_tiles("f64", cw, tg)
+            vals.append(em_pass(G, P, Q, loglik=True, stats=False).item())
+    finally:
+        autotune.set_tiles("f64", *default)
+    assert max(vals) - min(vals) <= 1e-13 * abs(want), vals
+    assert abs(vals[0] - want) <= 1e-12 * abs(want), (vals[0], want)
