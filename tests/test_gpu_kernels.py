@@ -185,6 +185,26 @@ def test_run_em_step_and_adam_step_match_oracle(cuda, precision):
         np.testing.assert_allclose(Qd.cpu().numpy(), Qo, rtol=1e-13, atol=1e-15)
 
 
+@pytest.mark.parametrize("setting", [(0.005, 0.80, 0.88, 1e-8), (0.05, 0.70, 0.95, 1e-3)])
+def test_adam_step_is_kingma_ba_textbook(cuda, setting):
+    """adx_adam_step (em_adam_gpu.py:11-42, :201-224) over three steps
+    against Kingma & Ba written out by hand (tests/textbook.py), on a case
+    whose steps cross both clamp bounds; P, Q and the moments to rounding."""
+    from tests import textbook as TB
+    lr, b1, b2, eps = setting
+    P0, Q0, P_t, Q_t = TB.adam_em_case()
+    M, K = P0.shape
+    N = Q0.shape[0]
+    opt = EMAdamOptimizer(P0.shape, Q0.shape, lr, b1, b2, eps, cuda, precision="f64")
+    Pd, Qd = torch.from_numpy(P0.copy()).to(cuda), torch.from_numpy(Q0.copy()).to(cuda)
+    for tp, tq in zip(P_t, Q_t):
+        opt.step(Pd, Qd, torch.from_numpy(tp).to(cuda), torch.from_numpy(tq).to(cuda))
+    ref = TB.adam_em_steps(P0, Q0, P_t, Q_t, lr, b1, b2, eps)
+    got = (Pd, Qd, opt.m_P, opt.v_P, opt.m_Q, opt.v_Q)
+    for name, g, r in zip(("P", "Q", "mP", "vP", "mQ", "vQ"), got, ref):
+        np.testing.assert_allclose(g.cpu().numpy(), np.array(r), rtol=1e-12, atol=1e-15, err_msg=name)
+
+
 @pytest.mark.parametrize("M,N,K,miss", SHAPES)
 def test_em_pass_bf16_error_vs_oracle(cuda, M, N, K, miss):
     """Mixed-precision form against the FP64 oracle at the same (P, Q).

@@ -217,6 +217,71 @@ def test_adamem_cpu_semantics_matches_reference_fixture(fixtures):
     assert abs(O.loglik_cpu(G, P, Q) - O.loglik_cpu(G, Pe, Qe)) <= 50.0
 
 
+# ------------------------------------------------------------ Adam step -----
+
+ADAM_SETTINGS = [(0.005, 0.80, 0.88, 1e-8),      # the reference's defaults (entry.py:35-47)
+                 (0.05, 0.70, 0.95, 1e-3)]
+
+
+def _oracle_adam_gpu(P0, Q0, P_t, Q_t, lr, b1, b2, eps):
+    M, K = P0.shape
+    opt = O.AdamEMGPU(M, Q0.shape[0], K, lr, b1, b2, eps)
+    P, Q = P0.copy(), Q0.copy()
+    for tp, tq in zip(P_t, Q_t):
+        opt.step(P, Q, tp, tq)
+    return P, Q, opt.mP, opt.vP, opt.mQ, opt.vQ
+
+
+def _oracle_adam_cpu(P0, Q0, P_t, Q_t, lr, b1, b2, eps):
+    from oracle import oracle as OO
+    M, K = P0.shape
+    N = Q0.shape[0]
+    P, Q = P0.copy(), Q0.copy()
+    mP, vP, mQ, vQ = np.zeros_like(P), np.zeros_like(P), np.zeros_like(Q), np.zeros_like(Q)
+    for t, (tp, tq) in enumerate(zip(P_t, Q_t), start=1):
+        tp, tq = np.ascontiguousarray(tp), np.ascontiguousarray(tq)
+        OO.lib().oc_adamP_cpu(OO._p(P), OO._p(tp), OO._p(mP), OO._p(vP), lr, b1, b2, eps, t, M, K)
+        OO.lib().oc_adamQ_cpu(OO._p(Q), OO._p(tq), OO._p(mQ), OO._p(vQ), lr, b1, b2, eps, t, N, K)
+    return P, Q, mP, vP, mQ, vQ
+
+
+@pytest.mark.parametrize("semantics", ["gpu", "cpu"])
+@pytest.mark.parametrize("setting", ADAM_SETTINGS)
+def test_adam_em_step_is_kingma_ba_textbook(semantics, setting):
+    """Three Adam-EM steps of the oracle (em_adam_gpu.py:11-42, :201-224 and
+    its Cython twin em.pyx:92-209) against Kingma & Ba written out by hand
+    (tests/textbook.py: closed-form moment sums, t incremented before use,
+    bias corrections, clamp of P and Q, then the Q row renormalisation), on
+    a case whose steps cross both clamp bounds.  P, Q and all four moment
+    buffers must agree to rounding."""
+    from tests import textbook as TB
+    lr, b1, b2, eps = setting
+    P0, Q0, P_t, Q_t = TB.adam_em_case()
+    run = _oracle_adam_gpu if semantics == "gpu" else _oracle_adam_cpu
+    got = run(P0, Q0, P_t, Q_t, lr, b1, b2, eps)
+    ref = TB.adam_em_steps(P0, Q0, P_t, Q_t, lr, b1, b2, eps)
+    for name, g, r in zip(("P", "Q", "mP", "vP", "mQ", "vQ"), got, ref):
+        np.testing.assert_allclose(g, np.array(r), rtol=1e-12, atol=1e-15, err_msg=name)
+    # the case exercises both bounds and the renormalisation
+    assert got[0].max() == TB.HI and got[0].min() == TB.LO
+    assert abs(got[1][0].sum() - 1.0) < 1e-15 and got[1][0, 1] < 1e-5
+
+
+@pytest.mark.parametrize("variant", ["no_bias_correction", "swapped_betas", "renorm_before_clamp",
+                                     "eps_inside_sqrt", "t_from_zero_at_step_two"])
+def test_adam_em_textbook_check_catches_planted_slips(variant):
+    """Each planted slip in the hand-written Adam step moves P or Q far
+    outside the 1e-12 tolerance of the check above, so an oracle (or a
+    kernel) carrying that slip would fail it."""
+    from tests import textbook as TB
+    lr, b1, b2, eps = ADAM_SETTINGS[1] if variant == "eps_inside_sqrt" else ADAM_SETTINGS[0]
+    P0, Q0, P_t, Q_t = TB.adam_em_case()
+    got = _oracle_adam_gpu(P0, Q0, P_t, Q_t, lr, b1, b2, eps)
+    bad = TB.adam_em_steps(P0, Q0, P_t, Q_t, lr, b1, b2, eps, variant=variant)
+    dev = max(np.abs(got[0] - np.array(bad[0])).max(), np.abs(got[1] - np.array(bad[1])).max())
+    assert dev > 1e-9, f"{variant}: deviation {dev:.2e} would pass the 1e-12 check"
+
+
 # ------------------------------------------------------------ init ----------
 
 def _centred_rows(G):
