@@ -60,6 +60,9 @@ def _tn(X: torch.Tensor, Y: torch.Tensor, alpha: float = 1.0) -> torch.Tensor:
     workspace: fixed summation order, so ALS runs are reproducible)."""
     n, a = X.shape
     b = Y.shape[1]
+    bc = min(64, 1024 // a)          # the kernel takes a, b <= 64 and a b <= 1024
+    if b > bc:                       # column blocks of Y (strided views, no copies)
+        return torch.cat([_tn(X, Y[:, c:c + bc], alpha) for c in range(0, b, bc)], dim=1)
     C = torch.empty((a, b), dtype=torch.float64, device=X.device)
     _lib.call("adx_gemm_tn_f64", X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0), n, a, b, alpha,
               C.data_ptr(), b, _tn_work(X.device, a, b).data_ptr(), _lib.stream_handle(X.device))
@@ -72,9 +75,13 @@ def _nn(X: torch.Tensor, W: torch.Tensor, alpha: float = 1.0, u: torch.Tensor | 
     n, a = X.shape
     b = W.shape[1]
     W = W.contiguous()
+    v = None if v is None else v.contiguous()
     C = torch.empty((n, b), dtype=torch.float64, device=X.device)
-    _lib.call("adx_gemm_nn_f64", X.data_ptr(), X.stride(0), n, a, b, W.data_ptr(), b, alpha, _lib.ptr(u),
-              _lib.ptr(None if v is None else v.contiguous()), 0.0, C.data_ptr(), b, _lib.stream_handle(X.device))
+    for c in range(0, b, 16):        # the kernel takes b <= 16: column blocks of W, v and C
+        bc = min(16, b - c)
+        _lib.call("adx_gemm_nn_f64", X.data_ptr(), X.stride(0), n, a, bc, W[:, c:].data_ptr(), b, alpha,
+                  _lib.ptr(u), _lib.ptr(None if v is None else v[c:]), 0.0, C[:, c:].data_ptr(), b,
+                  _lib.stream_handle(X.device))
     return C
 
 

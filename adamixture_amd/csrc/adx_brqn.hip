@@ -224,6 +224,165 @@ __global__ __launch_bounds__(256) void grad_hess_p_kernel(const uint8_t* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// K = 17 .. 64: one problem (sample or SNP) per wavefront.  K(K+1)/2 FP64
+// Hessian terms do not fit one thread's registers, so lane l of the wave owns
+// the 4 x 4 blocks t = l, l + 64, l + 128 of the upper block triangle (at most
+// 136 blocks at K = 64) and lanes 0 .. 15 own 4 gradient terms each.  The
+// four problems of a workgroup share one LDS-staged chunk of 64 operand rows
+// (P rows on the Q side, Q rows on the P side): lane j of a wave forms H and
+// the weights of row j of the chunk for its own problem, then every lane
+// accumulates its blocks over the 64 rows.  Each output element is written
+// by exactly one lane: no atomics, fixed summation order.
+constexpr int GHW_CH = 64;            // operand rows per chunk
+constexpr int GHW_LD = 65;            // LDS row stride (doubles): the H dot products of a wave hit distinct banks
+constexpr int GHW_TASKS = 3;          // 4 x 4 blocks per lane (ceil(136 / 64))
+
+template <int SIDE>
+__global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                             int64_t N, int K, const double* __restrict__ Q,
+                                                             const double* __restrict__ P, double* __restrict__ XtX,
+                                                             double* __restrict__ Xtz) {
+    __shared__ double s_op[GHW_CH * GHW_LD];   // chunk operand rows (zero past K and past the end)
+    __shared__ double s_own[4][64];            // each wave's own row (q_i or p_j)
+    __shared__ double s_w[4][4][GHW_CH];       // per wave: weights of the chunk rows
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t nprob = SIDE == 0 ? N : M;   // problems: samples (Q side) or SNPs (P side)
+    const int64_t nrow = SIDE == 0 ? M : N;    // reduction rows: SNPs or samples
+    const double* own_src = SIDE == 0 ? Q : P;
+    const double* op_src = SIDE == 0 ? P : Q;
+    const int64_t prob = (int64_t)blockIdx.x * 4 + wave;
+    const bool live = prob < nprob;
+    const int nb = (K + 3) >> 2;
+    const int ntask = nb * (nb + 1) / 2;
+    s_own[wave][lane] = (live && lane < K) ? own_src[prob * K + lane] : 0.0;
+    // lane's blocks (bk, bl), bk <= bl, enumerated row by row
+    int tk[GHW_TASKS], tl[GHW_TASKS];
+#pragma unroll
+    for (int u = 0; u < GHW_TASKS; ++u) {
+        int t = lane + 64 * u, bk = 0;
+        while (bk < nb && t >= nb - bk) {
+            t -= nb - bk;
+            ++bk;
+        }
+        tk[u] = bk < nb ? 4 * bk : -1;
+        tl[u] = bk < nb ? 4 * (bk + t) : -1;
+    }
+    double hx[GHW_TASKS][16];
+    double hz[4];
+#pragma unroll
+    for (int u = 0; u < GHW_TASKS; ++u)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) hx[u][e] = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) hz[a] = 0.0;
+
+    for (int64_t r0 = 0; r0 < nrow; r0 += GHW_CH) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < GHW_CH * 64; e += 256) {
+            const int r = e >> 6, k = e & 63;
+            s_op[r * GHW_LD + k] = (r0 + r < nrow && k < K) ? op_src[(r0 + r) * K + k] : 0.0;
+        }
+        __syncthreads();
+        // weights of row lane of the chunk for this wave's problem
+        {
+            const int64_t r = r0 + lane;
+            uint32_t code = 3u;
+            if (live && r < nrow) {
+                const int64_t snp = SIDE == 0 ? r : prob, smp = SIDE == 0 ? prob : r;
+                code = (G[(snp >> 2) * ldg + smp] >> (2 * (snp & 3))) & 3u;
+            }
+            double h = 0.0;
+            for (int k = 0; k < K; ++k) h += s_own[wave][k] * s_op[lane * GHW_LD + k];
+            const Weights w = bq_weights(h, code);
+            if (SIDE == 0) {
+                s_w[wave][0][lane] = w.z1;
+                s_w[wave][1][lane] = w.z2;
+                s_w[wave][2][lane] = w.w1;
+                s_w[wave][3][lane] = w.w2;
+            } else {
+                s_w[wave][0][lane] = w.z1 - w.z2;
+                s_w[wave][1][lane] = w.w1 + w.w2;
+            }
+        }
+        __syncthreads();
+        const int nr = (int)(nrow - r0 < GHW_CH ? nrow - r0 : GHW_CH);
+        for (int j = 0; j < nr; ++j) {
+            const double* x = s_op + j * GHW_LD;
+            if (SIDE == 0) {
+                // XtX_q += w1 p p^T + w2 (1-p)(1-p)^T, Xtz_q += z1 p + z2 (1-p)
+                const double z1 = s_w[wave][0][j], z2 = s_w[wave][1][j];
+                const double w1 = s_w[wave][2][j], w2 = s_w[wave][3][j];
+                if (lane < 16)
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const double pk = x[(4 * lane + a) & 63];
+                        hz[a] += z1 * pk + z2 * (1.0 - pk);
+                    }
+#pragma unroll
+                for (int u = 0; u < GHW_TASKS; ++u) {
+                    if (tk[u] < 0) continue;
+                    double ak[4], bk[4], pl[4], ul[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const double pk = x[tk[u] + a];
+                        ak[a] = w1 * pk;
+                        bk[a] = w2 * (1.0 - pk);
+                        pl[a] = x[tl[u] + a];
+                        ul[a] = 1.0 - pl[a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) hx[u][4 * a + b] += ak[a] * pl[b] + bk[a] * ul[b];
+                }
+            } else {
+                // XtX_p += (w1 + w2) q q^T, Xtz_p += (z1 - z2) q
+                const double zd = s_w[wave][0][j], ws = s_w[wave][1][j];
+                if (lane < 16)
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) hz[a] += zd * x[(4 * lane + a) & 63];
+#pragma unroll
+                for (int u = 0; u < GHW_TASKS; ++u) {
+                    if (tk[u] < 0) continue;
+                    double wk[4], ql[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        wk[a] = ws * x[tk[u] + a];
+                        ql[a] = x[tl[u] + a];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) hx[u][4 * a + b] += wk[a] * ql[b];
+                }
+            }
+        }
+    }
+    if (!live) return;
+    if (lane < 16)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+            if (4 * lane + a < K) Xtz[prob * K + 4 * lane + a] = hz[a];
+    double* H = XtX + prob * (int64_t)K * K;
+#pragma unroll
+    for (int u = 0; u < GHW_TASKS; ++u) {
+        if (tk[u] < 0) continue;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int k = tk[u] + a, l = tl[u] + b;
+                if (k < K && l < K) {
+                    H[k * K + l] = hx[u][4 * a + b];
+                    H[l * K + k] = hx[u][4 * a + b];
+                }
+            }
+    }
+    (void)ntask;
+}
+
 template <int KM>
 static void launch_gh(int side, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
                       const double* Q, const double* P, double* XtX, double* Xtz) {
@@ -257,12 +416,21 @@ extern "C" {
 int adx_grad_hess(int side, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const double* Q,
                   const double* P, double* XtX, double* Xtz, void* stream) {
     if (M <= 0 || N <= 0) return 0;
-    if (K < 1 || K > 16) return fail_msg(1, "adx_grad_hess: K must be in [1, 16]");
+    if (K < 1 || K > 64) return fail_msg(1, "adx_grad_hess: K must be in [1, 64]");
     if (side != 0 && side != 1) return fail_msg(1, "adx_grad_hess: side must be 0 (Q) or 1 (P)");
     if (ldg < N || ldg % 16 != 0) return fail_msg(1, "adx_grad_hess: ldg must be >= N and a multiple of 16");
     if (reinterpret_cast<uintptr_t>(G) % 16 != 0) return fail_msg(1, "adx_grad_hess: G not 16-byte aligned");
     hipStream_t st = (hipStream_t)stream;
     const int64_t rows = side == 0 ? N : M;
+    if (K > 16) {   // every output element is stored once: no clearing needed
+        const unsigned grid = (unsigned)ceil_div(rows, 4);
+        if (side == 0)
+            grad_hess_wide_kernel<0><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, XtX, Xtz);
+        else
+            grad_hess_wide_kernel<1><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, XtX, Xtz);
+        ADX_LAUNCH_CHECK("adx_grad_hess");
+        return 0;
+    }
     ADX_CHECK(hipMemsetAsync(XtX, 0, sizeof(double) * (size_t)(rows * K * K), st));
     ADX_CHECK(hipMemsetAsync(Xtz, 0, sizeof(double) * (size_t)(rows * K), st));
     // register blocks padded to an even K (the K(K+1)/2 Hessian terms of

@@ -20,8 +20,13 @@
 // s0+16g .. s0+16g+15) of byte row 16t+c, so every genotype byte is read
 // once, coalesced, and supplies the four SNP column blocks cb = 4t..4t+3.
 // A and B stay in MFMA accumulators for the whole strip; T is reduced per
-// sub-tile across the 4 waves in LDS and added to global memory with one
-// atomic per (sample, k) per workgroup.
+// sub-tile across the 4 waves in LDS and stored into the workgroup's slice
+// of a partial buffer (summed in a fixed order afterwards).
+//
+// K > 16 (up to the reference GPU limit of 64): the output k range of a
+// launch is the 16-wide block kb0 .. kb0 + 15 (A^T, B^T, T tiles are 16
+// wide); H is formed over the full K (KS up to 16 MFMA steps) in every
+// launch, one launch per k block.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -73,7 +78,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
     int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
     T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart,
-    T* __restrict__ ABpart) {
+    T* __restrict__ ABpart, int kb0) {
     using MM = Mma16<T>;
     using acc_t = typename MM::acc;
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
 #pragma unroll
         for (int kk = 0; kk < 4; ++kk) {
             const int64_t snp2 = snp_w + 64 * t + 4 * (4 * kk + g) + f;
-            pT[cb][kk] = (STATS && snp2 < M && cl < K) ? P[snp2 * ldp + cl] : T(0);
+            pT[cb][kk] = (STATS && snp2 < M && kb0 + cl < K) ? P[snp2 * ldp + kb0 + cl] : T(0);
         }
     }
 
@@ -173,7 +178,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int64_t sT = s0 + 16 * g + 4 * rw + j;
-                qT[j] = (STATS && sT < N && cl < K) ? Q[sT * ldq + cl] : T(0);
+                qT[j] = (STATS && sT < N && kb0 + cl < K) ? Q[sT * ldq + kb0 + cl] : T(0);
             }
             acc_t accT = acc_t{0, 0, 0, 0};
             T sb[4] = {T(0), T(0), T(0), T(0)};
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                     v += __shfl_xor(v, 2);
                     v += __shfl_xor(v, 4);
                     v += __shfl_xor(v, 8);
-                    if (cl < K) s_tsum[wave][(16 * g + 4 * rw + j) * 17 + cl] = accT[j] + v;
+                    if (kb0 + cl < K) s_tsum[wave][(16 * g + 4 * rw + j) * 17 + cl] = accT[j] + v;
                 }
             }
         }
@@ -236,12 +241,12 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
                 const int kp = e & 15;
                 const int64_t smp = s0 + (e >> 4);
                 const int ex = (e >> 4) * 17 + kp;
-                if (kp < K && smp < N) {
+                if (kb0 + kp < K && smp < N) {
                     const T v = (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
                     if (Tpart)
-                        Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] = v;
+                        Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] = v;
                     else
-                        atomicAdd(&Tacc[smp * K + kp], v);
+                        atomicAdd(&Tacc[smp * K + kb0 + kp], v);
                 }
             }
             __syncthreads();
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
             if (snp < M) {
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) {
-                    const int kp = MM::row(g, jj);
+                    const int kp = kb0 + MM::row(g, jj);
                     if (kp < K) {
                         // With a workspace: this sample split's slice of the A / B
                         // partial sums (each element written by one workgroup),
@@ -433,11 +438,16 @@ struct EmGrid {
     int64_t nbx, splits, n_per_split;
 };
 
-static EmGrid em_grid(int dtype, int64_t M, int64_t N) {
+// K > 16 ("wide"): the A/B/T contractions run in 16-wide k blocks, one
+// launch per block (H over the full K in each, the log-likelihood in the
+// first only), at CW = 4.
+static bool em_wide(int K) { return K > 16; }
+
+static EmGrid em_grid(int dtype, int64_t M, int64_t N, int K) {
     const EmTiles tl = em_tiles(dtype);
     EmGrid gd;
-    gd.cw = tl.cw;
-    gd.nbx = ceil_div(M, 64 * tl.cw);
+    gd.cw = em_wide(K) ? 4 : tl.cw;
+    gd.nbx = ceil_div(M, 64 * gd.cw);
     const int64_t target = tl.target_wgs;
     int64_t s = ceil_div(target, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
@@ -452,18 +462,31 @@ static EmGrid em_grid(int dtype, int64_t M, int64_t N) {
 template <typename T, int KS, bool STATS, bool LL>
 static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M, int64_t N,
                       int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B, T* Tm,
-                      double* llp, T* Tpart, T* ABpart) {
+                      double* llp, T* Tpart, T* ABpart, int kb0) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
-    if constexpr (sizeof(T) == 4) {  // the FP64 form has a single tile width
+    if constexpr (sizeof(T) == 4 && KS <= 4) {  // the FP64 and wide forms have a single tile width
         if (gd.cw == 8) {
             em_pass_kernel<T, KS, 8, STATS, LL>
                 <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart,
-                                       ABpart);
+                                       ABpart, kb0);
             return;
         }
     }
     em_pass_kernel<T, KS, 4, STATS, LL>
-        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart, ABpart);
+        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart, ABpart, kb0);
+}
+
+// One launch per 16-wide k block (one for K <= 16); the log-likelihood
+// rides on the first.
+template <typename T, int KS, bool STATS, bool LL>
+static void launch_em_blocks(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t M,
+                             int64_t N, int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B,
+                             T* Tm, double* llp, T* Tpart, T* ABpart) {
+    launch_em<T, KS, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart, 0);
+    if constexpr (STATS)
+        for (int kb0 = 16; kb0 < K; kb0 += 16)
+            launch_em<T, KS, true, false>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, nullptr, Tpart,
+                                          ABpart, kb0);
 }
 
 template <typename T, bool STATS, bool LL>
@@ -472,12 +495,17 @@ static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int6
                         T* Tm, double* llp, T* Tpart) {
     // Workspace after the T slices: A slices then B slices, one per sample split.
     T* ABpart = Tpart ? Tpart + gd.nbx * N * (int64_t)K : nullptr;
-    switch ((K + 3) / 4) {
-        case 1: launch_em<T, 1, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
-        case 2: launch_em<T, 2, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
-        case 3: launch_em<T, 3, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
-        default: launch_em<T, 4, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart); break;
-    }
+#define ADX_EM_KS(ks) \
+    launch_em_blocks<T, ks, STATS, LL>(gd, st, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, Tm, llp, Tpart, ABpart)
+    const int ks = (K + 3) / 4;
+    if (ks <= 1) ADX_EM_KS(1);
+    else if (ks == 2) ADX_EM_KS(2);
+    else if (ks == 3) ADX_EM_KS(3);
+    else if (ks == 4) ADX_EM_KS(4);
+    else if (ks <= 8) ADX_EM_KS(8);      // H steps past K multiply zeros
+    else if (ks <= 12) ADX_EM_KS(12);
+    else ADX_EM_KS(16);
+#undef ADX_EM_KS
     if (STATS && Tpart) {
         const int64_t nk = N * K, mk = M * K;
         tpart_reduce_kernel<T><<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, Tm);
@@ -515,7 +543,7 @@ int adx_get_em_tiles(int dtype, int* cw, int64_t* target_wgs) {
 
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
     if (M <= 0 || N <= 0) return 0;
-    const EmGrid gd = em_grid(dtype, M, N);
+    const EmGrid gd = em_grid(dtype, M, N, K);
     const int64_t esz = dtype == ADX_F64 ? 8 : 4;
     const int64_t k = K > 0 ? K : 0;
     return ll_part_bytes(gd) + esz * (gd.nbx * N * k + 2 * gd.splits * M * k);
@@ -525,7 +553,7 @@ int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, 
                 int64_t ldp, const void* Q, int64_t ldq, void* A, void* B, void* T, int flags,
                 double* ll_out, void* workspace, int64_t workspace_bytes, void* stream) {
     if (M <= 0 || N <= 0) return fail_msg(1, "adx_em_pass: empty genotype matrix");
-    if (K < 1 || K > 16) return fail_msg(1, "adx_em_pass: K must be in [1, 16]");
+    if (K < 1 || K > 64) return fail_msg(1, "adx_em_pass: K must be in [1, 64]");
     if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64)
         return fail_msg(1, "adx_em_pass: ldg must be a multiple of 16 and >= N rounded up to 64");
     if (ldp < K || ldq < K) return fail_msg(1, "adx_em_pass: ldp/ldq < K");
@@ -535,7 +563,7 @@ int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, 
     if (!stats && !ll) return 0;
     if (ll && dtype != ADX_F64) return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ADX_F64");
     if (stats && (!A || !B || !T)) return fail_msg(1, "adx_em_pass: A, B, T required");
-    const EmGrid gd = em_grid(dtype, M, N);
+    const EmGrid gd = em_grid(dtype, M, N, K);
     double* llp = nullptr;
     if (ll) {
         if (!ll_out || !workspace || workspace_bytes < ll_part_bytes(gd))

@@ -74,7 +74,7 @@ template <int KH, int CW>
 __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
     const __bf16* __restrict__ Qx, const _Float16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
-    float* __restrict__ Tpart, float* __restrict__ ABpart) {
+    float* __restrict__ Tpart, float* __restrict__ ABpart, int kb0, int kbt) {
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
     constexpr int LX = 32 * KH;  // row length of Px / Qx
     constexpr int TS = 20;       // row stride (words) of the per-wave transpose tile
@@ -113,8 +113,8 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         for (int i = 0; i < 8; ++i) {
             const int64_t s2 = snp_w + 64 * t + 4 * (4 * g + (i >> 1)) + f;
             float x = 0.0f;
-            if (s2 < M && cl < K) {
-                const float p = (float)Px[s2 * LX + cl] + (float)Px[s2 * LX + K + cl];
+            if (s2 < M && kb0 + cl < K) {
+                const float p = (float)Px[s2 * LX + kb0 + cl] + (float)Px[s2 * LX + K + kb0 + cl];
                 x = (i & 1) ? 1.0f - p : p;
             }
             v[i] = (_Float16)x;
@@ -152,7 +152,10 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     __shared__ u32x4 s_qt[4 * 64];
     __shared__ u32x4 s_qx[4 * KH * 64];
     const int tid = threadIdx.x;
-    const _Float16* qt_src = Qt + (int64_t)(tid >> 3) * ldqt + 8 * (tid & 7);
+    // Qt rows of this k block: hi rows kb0 .. kb0 + 15, lo rows 16 kbt + kb0 ..
+    const int qt_r = tid >> 3;
+    const _Float16* qt_src =
+        Qt + (int64_t)(qt_r < 16 ? kb0 + qt_r : 16 * kbt + kb0 + qt_r - 16) * ldqt + 8 * (tid & 7);
     const int qt_dst = (2 * (tid & 1) + (tid >> 7)) * 64 + 16 * ((tid & 7) >> 1) + ((tid >> 3) & 15);
     int qx_row[KH], qx_col[KH], qx_dst[KH];
 #pragma unroll
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (cl < K) s_tsum[wave][(16 * g + 4 * (2 * tp + rr) + j) * 17 + cl] = accT[rr][j];
+                    if (kb0 + cl < K) s_tsum[wave][(16 * g + 4 * (2 * tp + rr) + j) * 17 + cl] = accT[rr][j];
         }
 
         __syncthreads();
@@ -298,8 +301,8 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
             const int kp = e & 15;
             const int64_t smp = s0 + (e >> 4);
             const int ex = (e >> 4) * 17 + kp;
-            if (kp < K && smp < N)
-                Tpart[((int64_t)blockIdx.x * N + smp) * K + kp] =
+            if (kb0 + kp < K && smp < N)
+                Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] =
                     (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
         }
         __syncthreads();
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         if (snp < M) {
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
-                const int kp = 4 * g + jj;
+                const int kp = kb0 + 4 * g + jj;
                 if (kp < K) {
                     // this sample split's slice of the A / B partial sums (each
                     // element written by one workgroup), summed in a fixed order
@@ -328,18 +331,14 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
 // Operand arrays of the pass from the FP64 master P (M x K) and Q (N x K):
 //   Px[m]  = [P_hi(0..K), P_lo, P_hi, 0 ...]    (32*KH bf16)
 //   Qx[i]  = 8 [Q_hi, Q_hi, Q_lo, 0 ...]        (32*KH bf16)
-//   Qt     = [hi rows 0..15 | lo rows 0..15] x ldqt, Qt[kp][i] = 256 Q[i][kp]
-//            split into two FP16 terms
+//   Qt     = [hi rows 0..16KB-1 | lo rows 0..16KB-1] x ldqt, Qt[kp][i] = 256 Q[i][kp]
+//            split into two FP16 terms (KB = ceil(K / 16) k blocks)
 // with x_hi = bf16(x), x_lo = bf16(x - x_hi) (FP16 for Qt); zero outside the
 // valid range.  The power-of-two factors are exact.
-template <int KH>
-__global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, __bf16* __restrict__ Px) {
-    constexpr int LX = 32 * KH;
+__global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, int LX, __bf16* __restrict__ Px) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    __bf16 row[LX];
-#pragma unroll
-    for (int e = 0; e < LX; ++e) row[e] = (__bf16)0.0f;
+    __bf16* row = Px + m * LX;
     for (int k = 0; k < K; ++k) {
         const float x = (float)P[m * K + k];
         const __bf16 hi = (__bf16)x;
@@ -348,20 +347,15 @@ __global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, _
         row[K + k] = lo;
         row[2 * K + k] = hi;
     }
-#pragma unroll
-    for (int e = 0; e < LX; ++e) Px[m * LX + e] = row[e];
+    for (int e = 3 * K; e < LX; ++e) row[e] = (__bf16)0.0f;
 }
 
-template <int KH>
-__global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, __bf16* __restrict__ Qx,
-                               _Float16* __restrict__ Qt, int64_t ldqt) {
-    constexpr int LX = 32 * KH;
+__global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, int LX, int KB,
+                               __bf16* __restrict__ Qx, _Float16* __restrict__ Qt, int64_t ldqt) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ldqt) return;
-    __bf16 row[LX];
-#pragma unroll
-    for (int e = 0; e < LX; ++e) row[e] = (__bf16)0.0f;
-    for (int k = 0; k < 16; ++k) {
+    __bf16* row = i < N ? Qx + i * LX : nullptr;
+    for (int k = 0; k < 16 * KB; ++k) {
         _Float16 thi = (_Float16)0.0f, tlo = (_Float16)0.0f;
         if (i < N && k < K) {
             const float x = (float)Q[i * K + k];
@@ -375,23 +369,27 @@ __global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, _
             tlo = (_Float16)(y - (float)thi);
         }
         Qt[(int64_t)k * ldqt + i] = thi;
-        Qt[(int64_t)(16 + k) * ldqt + i] = tlo;
+        Qt[(int64_t)(16 * KB + k) * ldqt + i] = tlo;
     }
-    if (i < N)
-#pragma unroll
-        for (int e = 0; e < LX; ++e) Qx[i * LX + e] = row[e];
+    if (row)
+        for (int e = 3 * K; e < LX; ++e) row[e] = (__bf16)0.0f;
 }
+
+// 32-element chunks of the H operand rows (3K terms) and 16-wide k blocks.
+static int bf16_kh(int K) { return K <= 10 ? 1 : K <= 21 ? 2 : K <= 32 ? 3 : K <= 42 ? 4 : 6; }
+static int bf16_kb(int K) { return (K + 15) / 16; }
 
 struct Bf16Grid {
     int cw;
     int64_t nbx, splits, n_per_split;
 };
 
-static Bf16Grid bf16_grid(int64_t M, int64_t N) {
+// K > 16: one launch per 16-wide k block at CW = 4 (H over the full K in each).
+static Bf16Grid bf16_grid(int64_t M, int64_t N, int K) {
     const EmTiles tl = em_tiles(ADX_BF16);
     Bf16Grid gd;
-    gd.cw = tl.cw;
-    gd.nbx = ceil_div(M, 64 * tl.cw);
+    gd.cw = K > 16 ? 4 : tl.cw;
+    gd.nbx = ceil_div(M, 64 * gd.cw);
     int64_t s = ceil_div(tl.target_wgs, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
     if (s > max_s) s = max_s;
@@ -426,12 +424,20 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
                         float* B, float* T, float* Tpart) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     float* ABpart = Tpart + gd.nbx * N * (int64_t)K;
-    if (gd.cw == 8)
-        em_pass_bf16_kernel<KH, 8><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, Tpart,
-                                                         ABpart);
-    else
-        em_pass_bf16_kernel<KH, 4><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, Tpart,
-                                                         ABpart);
+    const int kbt = bf16_kb(K);
+    for (int kb = 0; kb < kbt; ++kb) {
+        bool done = false;
+        if constexpr (KH <= 2) {
+            if (gd.cw == 8) {
+                em_pass_bf16_kernel<KH, 8><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split,
+                                                                 Tpart, ABpart, 16 * kb, kbt);
+                done = true;
+            }
+        }
+        if (!done)
+            em_pass_bf16_kernel<KH, 4><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split,
+                                                             Tpart, ABpart, 16 * kb, kbt);
+    }
     // slices hold T / 8 (a' = a / 8) and 32 A, 32 B (Qt = 256 Q)
     const int64_t nk = N * K, mk = M * K;
     tpart_reduce_kernel<<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, 8.0f, T);
@@ -442,9 +448,9 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
 
 // Byte offset of the partial-T slices in the operand buffer, and its size.
 static int64_t bf16_base_bytes(int64_t M, int64_t N, int K) {
-    const int kh = K <= 10 ? 1 : 2;
+    const int kh = bf16_kh(K);
     const int64_t ld = ceil_div(N, 64) * 64;
-    return ceil_div((int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 32 * ld), 256) * 256;
+    return ceil_div((int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 32 * bf16_kb(K) * ld), 256) * 256;
 }
 
 }  // namespace adx
@@ -457,28 +463,23 @@ int64_t adx_em_bf16_operand_bytes(int64_t M, int64_t N, int K, int64_t* ldqt) {
     const int64_t ld = ceil_div(N, 64) * 64;
     if (ldqt) *ldqt = ld;
     if (M <= 0 || N <= 0) return bf16_base_bytes(M > 0 ? M : 0, N > 0 ? N : 0, K);
-    const Bf16Grid gd = bf16_grid(M, N);
+    const Bf16Grid gd = bf16_grid(M, N, K);
     return bf16_base_bytes(M, N, K) + (int64_t)sizeof(float) * (gd.nbx * N * K + 2 * gd.splits * M * K);
 }
 
 int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int K, void* operands,
                      int64_t operand_bytes, void* stream) {
-    if (K < 1 || K > 16) return fail_msg(1, "adx_em_prep_bf16: K must be in [1, 16]");
+    if (K < 1 || K > 64) return fail_msg(1, "adx_em_prep_bf16: K must be in [1, 64]");
     int64_t ldqt = 0;
     if (operand_bytes < adx_em_bf16_operand_bytes(M, N, K, &ldqt))
         return fail_msg(1, "adx_em_prep_bf16: operand buffer too small");
-    const int kh = K <= 10 ? 1 : 2;
+    const int kh = bf16_kh(K);
     __bf16* Px = static_cast<__bf16*>(operands);
     __bf16* Qx = Px + M * 32 * kh;
     _Float16* Qt = reinterpret_cast<_Float16*>(Qx + N * 32 * kh);
     hipStream_t st = (hipStream_t)stream;
-    if (kh == 1) {
-        if (M > 0) prep_px_kernel<1><<<(unsigned)ceil_div(M, 256), 256, 0, st>>>(P, M, K, Px);
-        prep_qx_kernel<1><<<(unsigned)ceil_div(ldqt, 256), 256, 0, st>>>(Q, N, K, Qx, Qt, ldqt);
-    } else {
-        if (M > 0) prep_px_kernel<2><<<(unsigned)ceil_div(M, 256), 256, 0, st>>>(P, M, K, Px);
-        prep_qx_kernel<2><<<(unsigned)ceil_div(ldqt, 256), 256, 0, st>>>(Q, N, K, Qx, Qt, ldqt);
-    }
+    if (M > 0) prep_px_kernel<<<(unsigned)ceil_div(M, 256), 256, 0, st>>>(P, M, K, 32 * kh, Px);
+    prep_qx_kernel<<<(unsigned)ceil_div(ldqt, 256), 256, 0, st>>>(Q, N, K, 32 * kh, bf16_kb(K), Qx, Qt, ldqt);
     ADX_LAUNCH_CHECK("adx_em_prep_bf16");
     return 0;
 }
@@ -486,7 +487,7 @@ int adx_em_prep_bf16(const double* P, const double* Q, int64_t M, int64_t N, int
 int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, void* operands,
                      int64_t operand_bytes, float* A, float* B, float* T, void* stream) {
     if (M <= 0 || N <= 0) return fail_msg(1, "adx_em_pass_bf16: empty genotype matrix");
-    if (K < 1 || K > 16) return fail_msg(1, "adx_em_pass_bf16: K must be in [1, 16]");
+    if (K < 1 || K > 64) return fail_msg(1, "adx_em_pass_bf16: K must be in [1, 64]");
     if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64)
         return fail_msg(1, "adx_em_pass_bf16: ldg must be a multiple of 16 and >= N rounded up to 64");
     if (reinterpret_cast<uintptr_t>(G) % 16 != 0 || reinterpret_cast<uintptr_t>(operands) % 16 != 0)
@@ -494,17 +495,20 @@ int adx_em_pass_bf16(const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
     int64_t ldqt = 0;
     if (operand_bytes < adx_em_bf16_operand_bytes(M, N, K, &ldqt))
         return fail_msg(1, "adx_em_pass_bf16: operand buffer too small");
-    const int kh = K <= 10 ? 1 : 2;
+    const int kh = bf16_kh(K);
     const __bf16* Px = static_cast<const __bf16*>(operands);
     const __bf16* Qx = Px + M * 32 * kh;
     const _Float16* Qt = reinterpret_cast<const _Float16*>(Qx + N * 32 * kh);
     float* Tpart = reinterpret_cast<float*>(static_cast<uint8_t*>(operands) + bf16_base_bytes(M, N, K));
-    const Bf16Grid gd = bf16_grid(M, N);
+    const Bf16Grid gd = bf16_grid(M, N, K);
     hipStream_t st = (hipStream_t)stream;
-    if (kh == 1)
-        launch_bf16<1>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart);
-    else
-        launch_bf16<2>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart);
+    switch (kh) {
+        case 1: launch_bf16<1>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart); break;
+        case 2: launch_bf16<2>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart); break;
+        case 3: launch_bf16<3>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart); break;
+        case 4: launch_bf16<4>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart); break;
+        default: launch_bf16<6>(gd, st, G, ldg, M, N, K, Px, Qx, Qt, ldqt, A, B, T, Tpart); break;
+    }
     ADX_LAUNCH_CHECK("adx_em_pass_bf16");
     return 0;
 }

@@ -48,9 +48,16 @@ def _work(G: PackedGenotypes, which: int, L: int):
     return w
 
 
+# Columns per decode-fused GEMM launch (the kernels take L <= 64; the
+# rank-L test matrix, L = max(k + 10, 20), is wider for k > 54).
+MAX_L = 64
+
+
 def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
     """Y = X W (M x L) with X the centred genotypes, W (N x L) FP32."""
     L = W.shape[1]
+    if L > MAX_L:   # column blocks: each column of Y depends on one column of W only
+        return torch.cat([xw(G, f, W[:, c:c + MAX_L].contiguous()) for c in range(0, L, MAX_L)], dim=1)
     Y = torch.zeros((G.M, L), dtype=torch.float32, device=G.device)
     _lib.call("adx_svd_xw", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.contiguous().data_ptr(), L,
               Y.data_ptr(), _lib.ptr(_work(G, 0, L)), _lib.stream_handle(G.device))
@@ -60,6 +67,8 @@ def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
 def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
     """A = X^T Y (N x L) with Y (M x L) FP32."""
     L = Y.shape[1]
+    if L > MAX_L:
+        return torch.cat([xty(G, f, Y[:, c:c + MAX_L].contiguous()) for c in range(0, L, MAX_L)], dim=1)
     A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
     _lib.call("adx_svd_xty", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.contiguous().data_ptr(), L,
               A.data_ptr(), _lib.ptr(_work(G, 1, L)), _lib.stream_handle(G.device))

@@ -109,7 +109,10 @@ int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
 /* Replaces the chunk loop of EMAdamOptimizer.run_em_step with
  * em_batch_math (adamixture/model/em_adam_gpu.py:166-197, :46-78), and, with
  * ADX_EM_LOGLIK, logl_batch_math (:226-248) at the same (P, Q).
- * One pass over all genotypes; 1 <= K <= 16.
+ * One pass over all genotypes; 1 <= K <= 64, the reference GPU path's limit
+ * (adamixture/entry.py:299-303).  For K > 16 the A, B, T contractions run in
+ * 16-wide k blocks, one launch per block (H over the full K in each, the
+ * log-likelihood in the first) at a 4-tile width.
  *   ADX_EM_STATS:  A += a Q, B += b Q (M x K), T += (a-b)^T P + colsum(b)
  *                  (N x K), with H = P Q^T clamped to [1e-5, 1-1e-5],
  *                  a = g/H, b = (2-g)/(1-H) on observed genotypes.  A, B, T
@@ -125,7 +128,8 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
                 const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
                 int flags, double *ll_out, void *workspace, int64_t workspace_bytes, void *stream);
 
-/* Mixed-precision form of adx_em_pass (ADX_EM_STATS only, 1 <= K <= 16),
+/* Mixed-precision form of adx_em_pass (ADX_EM_STATS only, 1 <= K <= 64;
+ * K > 16 in 16-wide k blocks as adx_em_pass),
  * FP32 A, B, T (dtype tag ADX_BF16).  Same contract as adx_em_pass with
  * dtype ADX_F32, but the contractions run on 16x16x32 MFMA with FP32
  * accumulation from operand arrays that adx_em_prep_bf16 builds from the
@@ -208,7 +212,7 @@ int adx_svd_xty(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float
                 float *A, float *work, void *stream);
 int64_t adx_svd_work_floats(int which, int64_t M, int64_t N, int L);
 
-/* --- SQP block-relaxation gradient / Hessian (FP64, 1 <= K <= 16) ---------- */
+/* --- SQP block-relaxation gradient / Hessian (FP64, 1 <= K <= 64) ---------- */
 
 /* Replaces compute_grad_hess_Q_gpu (side 0) and compute_grad_hess_P_gpu
  * (side 1) with their chunk math (adamixture/model/br_qn_gpu.py:67-214):
@@ -219,7 +223,9 @@ int64_t adx_svd_work_floats(int which, int64_t M, int64_t N, int L);
  *           XtX (M x K x K) = sum_i (g/H^2 + (2-g)/(1-H)^2) q_i q_i^T
  * (positive gradient convention; the reference negates before the SQP
  * solve).  Outputs are overwritten.  G 16-byte aligned, ldg >= N and a
- * multiple of 16 (the padded device layout). */
+ * multiple of 16 (the padded device layout).  K <= 16: one thread per
+ * problem, register blocks; K > 16: one wavefront per problem, each lane
+ * owning 4 x 4 blocks of the Hessian (deterministic, no atomics). */
 int adx_grad_hess(int side, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K, const double *Q,
                   const double *P, double *XtX, double *Xtz, void *stream);
 
