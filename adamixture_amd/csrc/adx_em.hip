@@ -443,12 +443,12 @@ struct EmGrid {
 // first only), at CW = 4.
 static bool em_wide(int K) { return K > 16; }
 
-static EmGrid em_grid(int dtype, int64_t M, int64_t N, int K) {
+static EmGrid em_grid(int dtype, int64_t M, int64_t N, int K, int64_t target = 0) {
     const EmTiles tl = em_tiles(dtype);
     EmGrid gd;
     gd.cw = em_wide(K) ? 4 : tl.cw;
     gd.nbx = ceil_div(M, 64 * gd.cw);
-    const int64_t target = tl.target_wgs;
+    if (target <= 0) target = tl.target_wgs;
     int64_t s = ceil_div(target, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
     if (s > max_s) s = max_s;
@@ -515,9 +515,95 @@ static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int6
     }
 }
 
-// Workspace layout of adx_em_pass: [LL partials: nbx*splits doubles, padded
-// to 256 B][T partial slices: nbx x N x K of the pass dtype].
-static int64_t ll_part_bytes(const EmGrid& gd) { return ceil_div((int64_t)sizeof(double) * gd.nbx * gd.splits, 256) * 256; }
+// A pass over SNP bands.  Band b covers SNPs b*band .. min((b+1)*band, M) - 1
+// (band a multiple of the 64*cw SNPs of one workgroup column) and is one
+// launch sequence over the sub-matrix of its byte rows; its T / A / B
+// partial slices live in one region of the workspace reused band after band,
+// and its log-likelihood partials are summed into one slot per band.  The
+// band slots are summed in band order at the end, so for a given plan the
+// pass is deterministic.  Workspace layout:
+//   [LL partials of one band][LL band slots][T slices | A slices | B slices]
+struct EmPlan {
+    int64_t band, nbands, target;
+    int64_t ll_slots;      // LL partials of the largest band (its nbx x splits)
+    int64_t ll_bytes;      // both LL regions, 256-byte padded
+    int64_t slice_bytes;   // largest band's partial slices (0 without STATS)
+};
+
+static int64_t pad256(int64_t b) { return ceil_div(b, 256) * 256; }
+
+static EmPlan em_plan_for(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t band, int64_t target) {
+    EmPlan p;
+    p.band = band < M ? band : M;
+    p.nbands = ceil_div(M, p.band);
+    p.target = target;
+    const int64_t last = M - (p.nbands - 1) * p.band;
+    const EmGrid g1 = em_grid(dtype, p.band, N, K, target), g2 = em_grid(dtype, last, N, K, target);
+    const int64_t s1 = g1.nbx * g1.splits, s2 = g2.nbx * g2.splits;
+    p.ll_slots = s1 > s2 ? s1 : s2;
+    p.ll_bytes = pad256(8 * p.ll_slots) + pad256(8 * p.nbands);
+    const int64_t esz = dtype == ADX_F64 ? 8 : 4;
+    const int64_t b1 = esz * K * (g1.nbx * N + 2 * g1.splits * p.band);
+    const int64_t b2 = esz * K * (g2.nbx * N + 2 * g2.splits * last);
+    p.slice_bytes = stats ? pad256(b1 > b2 ? b1 : b2) : 0;
+    return p;
+}
+
+static int64_t em_plan_bytes(const EmPlan& p) { return p.ll_bytes + p.slice_bytes; }
+
+// The plan for a workspace of ws bytes: one band if it fits, else the widest
+// band (then the largest workgroup target, halved down to 1: fewer sample
+// splits, fewer A / B slices) that fits.
+// False if none does.
+static bool em_plan(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t ws, EmPlan& out) {
+    const EmTiles tl = em_tiles(dtype);
+    out = em_plan_for(dtype, M, N, K, stats, M, tl.target_wgs);
+    if (em_plan_bytes(out) <= ws) return true;
+    const int64_t unit = 64 * (em_wide(K) ? 4 : tl.cw);
+    const int64_t cmax = ceil_div(M, unit);
+    for (int64_t target = tl.target_wgs; target >= 1; target /= 2) {
+        // the slices grow with the band: the widest fitting band by bisection
+        int64_t lo = 0, hi = cmax;   // fits(lo) (or lo = 0), !fits(hi)
+        while (hi - lo > 1) {
+            const int64_t c = (lo + hi) / 2;
+            if (em_plan_bytes(em_plan_for(dtype, M, N, K, stats, c * unit, target)) <= ws)
+                lo = c;
+            else
+                hi = c;
+        }
+        if (lo >= 1) {
+            out = em_plan_for(dtype, M, N, K, stats, lo * unit, target);
+            if (em_plan_bytes(out) <= ws) return true;
+        }
+    }
+    return false;
+}
+
+// The pass over one band (sub-matrix of byte rows m0 / 4 ..): G, P, A, B
+// offset to the band, Q and T whole.
+template <typename T, bool STATS, bool LL>
+static void launch_band(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_t ldg, int64_t m0, int64_t Mb,
+                        int64_t N, int K, const void* P, int64_t ldp, const void* Q, int64_t ldq, void* A, void* B,
+                        void* Tm, double* llp, void* tpart) {
+    const T* Pb = static_cast<const T*>(P) + m0 * ldp;
+    T* Ab = STATS ? static_cast<T*>(A) + m0 * K : nullptr;
+    T* Bb = STATS ? static_cast<T*>(B) + m0 * K : nullptr;
+    launch_em_k<T, STATS, LL>(gd, st, G + (m0 >> 2) * ldg, ldg, Mb, N, K, Pb, ldp, static_cast<const T*>(Q), ldq,
+                              Ab, Bb, STATS ? static_cast<T*>(Tm) : nullptr, llp, static_cast<T*>(tpart));
+}
+
+static void launch_any(int dtype, bool stats, bool ll, const EmGrid& gd, hipStream_t st, const uint8_t* G,
+                       int64_t ldg, int64_t m0, int64_t Mb, int64_t N, int K, const void* P, int64_t ldp,
+                       const void* Q, int64_t ldq, void* A, void* B, void* Tm, double* llp, void* tpart) {
+    if (dtype == ADX_F32)
+        launch_band<float, true, false>(gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, Tm, nullptr, tpart);
+    else if (stats && ll)
+        launch_band<double, true, true>(gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, Tm, llp, tpart);
+    else if (stats)
+        launch_band<double, true, false>(gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, Tm, nullptr, tpart);
+    else
+        launch_band<double, false, true>(gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, Tm, llp, nullptr);
+}
 
 }  // namespace adx
 
@@ -543,10 +629,8 @@ int adx_get_em_tiles(int dtype, int* cw, int64_t* target_wgs) {
 
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
     if (M <= 0 || N <= 0) return 0;
-    const EmGrid gd = em_grid(dtype, M, N, K);
-    const int64_t esz = dtype == ADX_F64 ? 8 : 4;
-    const int64_t k = K > 0 ? K : 0;
-    return ll_part_bytes(gd) + esz * (gd.nbx * N * k + 2 * gd.splits * M * k);
+    // K = 0: the log-likelihood partials only
+    return em_plan_bytes(em_plan_for(dtype, M, N, K > 0 ? K : 1, K > 0, M, em_tiles(dtype).target_wgs));
 }
 
 int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
@@ -558,44 +642,40 @@ int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, 
         return fail_msg(1, "adx_em_pass: ldg must be a multiple of 16 and >= N rounded up to 64");
     if (ldp < K || ldq < K) return fail_msg(1, "adx_em_pass: ldp/ldq < K");
     if (reinterpret_cast<uintptr_t>(G) % 16 != 0) return fail_msg(1, "adx_em_pass: G not 16-byte aligned");
+    if (dtype != ADX_F32 && dtype != ADX_F64) return fail_msg(1, "adx_em_pass: dtype must be ADX_F32 or ADX_F64");
     const bool stats = flags & ADX_EM_STATS;
     const bool ll = flags & ADX_EM_LOGLIK;
     if (!stats && !ll) return 0;
     if (ll && dtype != ADX_F64) return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ADX_F64");
     if (stats && (!A || !B || !T)) return fail_msg(1, "adx_em_pass: A, B, T required");
-    const EmGrid gd = em_grid(dtype, M, N, K);
-    double* llp = nullptr;
-    if (ll) {
-        if (!ll_out || !workspace || workspace_bytes < ll_part_bytes(gd))
-            return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ll_out and adx_em_workspace_bytes of workspace");
-        llp = static_cast<double*>(workspace);
-    }
-    // T partial slices when the workspace holds them, else atomics.
-    void* tpart = nullptr;
-    if (stats && workspace && workspace_bytes >= adx_em_workspace_bytes(dtype, M, N, K))
-        tpart = static_cast<uint8_t*>(workspace) + ll_part_bytes(gd);
+    if (ll && !ll_out) return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ll_out");
     hipStream_t st = (hipStream_t)stream;
-    if (dtype == ADX_F32) {
-        launch_em_k<float, true, false>(gd, st, G, ldg, M, N, K, (const float*)P, ldp, (const float*)Q, ldq,
-                                        (float*)A, (float*)B, (float*)T, nullptr, (float*)tpart);
-    } else if (dtype == ADX_F64) {
-        const double* Pd = (const double*)P;
-        const double* Qd = (const double*)Q;
-        if (stats && ll)
-            launch_em_k<double, true, true>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, (double*)A, (double*)B,
-                                            (double*)T, llp, (double*)tpart);
-        else if (stats)
-            launch_em_k<double, true, false>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, (double*)A, (double*)B,
-                                             (double*)T, nullptr, (double*)tpart);
-        else
-            launch_em_k<double, false, true>(gd, st, G, ldg, M, N, K, Pd, ldp, Qd, ldq, nullptr, nullptr,
-                                             nullptr, llp, nullptr);
-    } else {
-        return fail_msg(1, "adx_em_pass: dtype must be ADX_F32 or ADX_F64");
+    EmPlan plan;
+    const bool planned = workspace && em_plan(dtype, M, N, K, stats, workspace_bytes, plan);
+    if (!planned) {
+        if (ll)
+            return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs a workspace (adx_em_workspace_bytes, or enough "
+                               "for one SNP band)");
+        // no workspace: T, A, B by device-scope atomics (order varies run to run)
+        launch_any(dtype, true, false, em_grid(dtype, M, N, K), st, G, ldg, 0, M, N, K, P, ldp, Q, ldq, A, B, T,
+                   nullptr, nullptr);
+        ADX_LAUNCH_CHECK("adx_em_pass");
+        return 0;
     }
-    ADX_LAUNCH_CHECK("adx_em_pass");
+    uint8_t* ws = static_cast<uint8_t*>(workspace);
+    double* llp = reinterpret_cast<double*>(ws);
+    double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
+    void* tpart = stats ? ws + plan.ll_bytes : nullptr;
+    for (int64_t b = 0; b < plan.nbands; ++b) {
+        const int64_t m0 = b * plan.band;
+        const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
+        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.target);
+        launch_any(dtype, stats, ll, gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, T, llp, tpart);
+        if (ll) reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, slots + b);
+        ADX_LAUNCH_CHECK("adx_em_pass");
+    }
     if (ll) {
-        reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, ll_out);
+        reduce_f64_kernel<<<1, 256, 0, st>>>(slots, plan.nbands, ll_out);
         ADX_LAUNCH_CHECK("adx_em_pass(reduce)");
     }
     return 0;

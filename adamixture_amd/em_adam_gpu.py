@@ -47,11 +47,29 @@ def _check_params(P: torch.Tensor, Q: torch.Tensor) -> None:
 
 
 class _Workspace:
-    """Per-device scratch for the log-likelihood partial sums."""
+    """Per-device scratch of the F32/F64 pass: log-likelihood partial sums
+    and the T / A / B partial slices.  The pass needs the single-band size
+    (adx_em_workspace_bytes) to run in one band; given less it runs in SNP
+    bands whose slices fit.  The buffer is capped at ``cap(G)``: the larger
+    of a tenth of the packed genotype matrix and ``min_cap`` bytes."""
 
     def __init__(self):
         self.buf: torch.Tensor | None = None
         self.ll = None
+        self.min_cap = 256 << 20
+        self.frac = 0.1
+
+    def cap(self, G: PackedGenotypes) -> int:
+        return max(int(self.frac * G.data.numel()), self.min_cap)
+
+    def shrink(self) -> bool:
+        """Halve the cap after an out-of-memory error (False at the floor)."""
+        if self.min_cap <= (32 << 20) and self.frac <= 1e-3:
+            return False
+        self.min_cap = max(self.min_cap // 2, 32 << 20)
+        self.frac = max(self.frac / 2, 1e-3)
+        self.buf = None
+        return True
 
     def get(self, device, nbytes: int) -> tuple[torch.Tensor, torch.Tensor]:
         if self.buf is None or self.buf.numel() < nbytes or self.buf.device != device:
@@ -72,7 +90,7 @@ def em_pass(G: PackedGenotypes, P, Q, A=None, B=None, T=None, *, dtype: int = _l
     K = P.shape[1]
     flags = (_lib.ADX_EM_STATS if stats else 0) | (_lib.ADX_EM_LOGLIK if loglik else 0)
     # K = 0 sizes only the log-likelihood partials (no T slices without stats)
-    ws_bytes = _lib.load().adx_em_workspace_bytes(dtype, G.M, G.N, K if stats else 0)
+    ws_bytes = min(_lib.load().adx_em_workspace_bytes(dtype, G.M, G.N, K if stats else 0), _WS.cap(G))
     ws, ll = _WS.get(G.device, ws_bytes)
     _lib.call("adx_em_pass", dtype, G.data.data_ptr(), G.ldg, G.M, G.N, K, P.data_ptr(), P.stride(0),
               Q.data_ptr(), Q.stride(0), _lib.ptr(A), _lib.ptr(B), _lib.ptr(T), flags,
@@ -279,6 +297,14 @@ def optimize_parameters_gpu(G, P, Q, lr, beta1, beta2, reg_adam, max_iter, check
                                                  check, K, M, N, lr_decay, min_lr, patience, tol_adam, device,
                                                  cur, threads_per_block, precision, on_check, after_step)
         except RuntimeError as exc:
-            if not is_gpu_oom(exc) or cur <= MIN_GPU_CHUNK_SIZE:
+            if not is_gpu_oom(exc):
+                raise
+            # the chunk size is the reference's knob (no chunk temporaries
+            # here); what can shrink is the pass's partial-sum workspace
+            if _WS.shrink():
+                log.warning("    GPU out of memory while running Adam-EM: retrying with the EM pass's "
+                            f"workspace capped at {_WS.min_cap >> 20} MiB.")
+                continue
+            if cur <= MIN_GPU_CHUNK_SIZE:
                 raise
             cur = reduce_gpu_chunk_or_raise(cur, exc, "running Adam-EM")

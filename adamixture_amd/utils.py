@@ -119,8 +119,11 @@ def estimate_gpu_bytes(M: int, N: int, K: int, algorithm: str, q_hist: int = 3,
     svd = M * L * 4 + 2 * N * L * 4 + M * L * 4
     als = 5 * (M + N) * K * 8
     if algorithm == "adamem":
-        # + the per-workgroup T partial sums of the FP64 pass (256 SNPs per slice)
-        opt = (8 * M * K + 7 * N * K + N) * 8 + 2 * (M + N) * K * 4 + ((M + 255) // 256) * N * K * 8
+        # + the FP64 pass's partial-sum workspace: the single-band T, A and B
+        # slices, capped at max(G / 10, 256 MiB) (the pass then runs in SNP bands)
+        g_bytes = ((M + 3) // 4) * ((N + 63) // 64 * 64)
+        ws = min(_lib.load().adx_em_workspace_bytes(_lib.ADX_F64, M, N, K), max(g_bytes // 10, 256 << 20))
+        opt = (8 * M * K + 7 * N * K + N) * 8 + 2 * (M + N) * K * 4 + ws
     elif algorithm == "brqn":
         dim = (M + N) * K
         opt = ((M + N) * K * K + (M + N) * K) * 8 + 3 * dim * q_hist * 8 + 9 * dim * 8

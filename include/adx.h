@@ -100,10 +100,12 @@ int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int
 int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs);
 int adx_get_em_tiles(int dtype, int *cw, int64_t *target_wgs);
 
-/* Workspace (bytes) of adx_em_pass for the given shape, dtype and current
- * tile shape: the log-likelihood partial sums, the per-workgroup partial sums
- * of T (ceil(M / (64 cw)) x N x K of the pass dtype) and the per-sample-split
- * partial sums of A and B (2 x splits x M x K). */
+/* Workspace (bytes) for adx_em_pass to run in one SNP band for the given
+ * shape, dtype and current tile shape: the log-likelihood partial sums, the
+ * per-workgroup partial sums of T (ceil(M / (64 cw)) x N x K of the pass
+ * dtype) and the per-sample-split partial sums of A and B (2 x splits x M x
+ * K).  K = 0 sizes the log-likelihood partials only.  Given less, the pass
+ * runs in SNP bands (below). */
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
 
 /* Replaces the chunk loop of EMAdamOptimizer.run_em_step with
@@ -119,10 +121,16 @@ int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
  *                  are accumulated (caller zeroes them; adx_em_finalize does).
  *                  With a workspace of adx_em_workspace_bytes, T, A and B
  *                  are reduced through partial sums (plain stores, summed in
- *                  a fixed order: deterministic); with less, by device-scope
- *                  atomics.
+ *                  a fixed order: deterministic).  With less, the pass runs
+ *                  over SNP bands (multiples of one workgroup column) whose
+ *                  partial sums fit, band after band in a fixed order, with
+ *                  fewer sample splits if one column needs it: deterministic
+ *                  for a given workspace size, equal to the one-band result
+ *                  up to summation order.  With too little for one band (or
+ *                  none), T, A, B by device-scope atomics (order varies).
  *   ADX_EM_LOGLIK: *ll_out (device, FP64) = sum g log H + (2-g) log1p(-H),
- *                  H clamped to [1e-10, 1-1e-10]; FP64 dtype only.
+ *                  H clamped to [1e-10, 1-1e-10]; FP64 dtype only; needs a
+ *                  workspace of at least one band (an error otherwise).
  * dtype selects the arithmetic of P, Q, A, B, T: ADX_F64 or ADX_F32. */
 int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K,
                 const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,

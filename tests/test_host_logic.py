@@ -32,7 +32,11 @@ def test_estimate_gpu_bytes_orders_algorithms():
     a = utils.estimate_gpu_bytes(M, N, K, "adamem")
     b = utils.estimate_gpu_bytes(M, N, K, "brqn")
     assert a > 0 and b >= ((M + N) * K * K) * 8          # BR-QN keeps (M+N) K^2 FP64 Hessians
-    assert a >= ((M + 255) // 256) * N * K * 8           # Adam-EM keeps the FP64 T partial sums
+    # Adam-EM keeps the FP64 pass's partial sums, capped at max(G / 10, 256 MiB)
+    g_bytes = ((M + 3) // 4) * ((N + 63) // 64 * 64)
+    cap = max(g_bytes // 10, 256 << 20)
+    assert a >= (8 * M * K + 7 * N * K) * 8 + cap
+    assert a <= (8 * M * K + 7 * N * K + N) * 8 + 2 * (M + N) * K * 4 + cap + (M * 20 * 4 * 3)
     assert utils.estimate_gpu_bytes(M, N, K, "adamem", include_initialization=False) <= a
     assert utils.estimate_gpu_bytes(2 * M, N, K, "adamem") > a
     with pytest.raises(ValueError):
