@@ -22,6 +22,7 @@ ADX_EM_STATS = 1
 ADX_EM_LOGLIK = 2
 ADX_CV_BLOCKS = 4096          # include/adx.h
 ADX_GEMM_TN_BLOCKS = 512      # include/adx.h
+ADX_EM_NOALLOC_SCRATCH_BYTES = 128 << 20   # include/adx.h
 
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -42,6 +43,8 @@ SIGNATURES = {
     "adx_em_workspace_bytes": (_i64, [_int, _i64, _i64, _int]),
     "adx_em_pass": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _i64, _p, _i64, _p, _p, _p, _int, _p,
                            _p, _i64, _p]),
+    "adx_em_pass_noalloc": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _i64, _p, _i64, _p, _p, _p, _int,
+                                   _p, _p]),
     "adx_em_finalize": (_int, [_int, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _p, _p]),
     "adx_adam_step": (_int, [_p, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _dbl, _dbl, _dbl, _dbl,
                              _dbl, _p, _p, _i64, _p]),

@@ -31,6 +31,9 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+#include <string>
+
 #include "adx_common.h"
 #include "../../include/adx.h"
 
@@ -605,6 +608,100 @@ static void launch_any(int dtype, bool stats, bool ll, const EmGrid& gd, hipStre
         launch_band<double, false, true>(gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, Tm, llp, nullptr);
 }
 
+// Argument checks of adx_em_pass (-1 if the call can go ahead; 0 if there is
+// nothing to do; else the failure status).
+static int em_pass_check(const char* name, int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
+                         int64_t ldp, int64_t ldq, const void* A, const void* B, const void* T, int flags,
+                         const double* ll_out) {
+    const std::string who(name);
+    if (M <= 0 || N <= 0) return fail_msg(1, who + ": empty genotype matrix");
+    if (K < 1 || K > 64) return fail_msg(1, who + ": K must be in [1, 64]");
+    if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64)
+        return fail_msg(1, who + ": ldg must be a multiple of 16 and >= N rounded up to 64");
+    if (ldp < K || ldq < K) return fail_msg(1, who + ": ldp/ldq < K");
+    if (reinterpret_cast<uintptr_t>(G) % 16 != 0) return fail_msg(1, who + ": G not 16-byte aligned");
+    if (dtype != ADX_F32 && dtype != ADX_F64) return fail_msg(1, who + ": dtype must be ADX_F32 or ADX_F64");
+    const bool stats = flags & ADX_EM_STATS;
+    const bool ll = flags & ADX_EM_LOGLIK;
+    if (!stats && !ll) return 0;
+    if (ll && dtype != ADX_F64) return fail_msg(1, who + ": ADX_EM_LOGLIK needs ADX_F64");
+    if (stats && (!A || !B || !T)) return fail_msg(1, who + ": A, B, T required");
+    if (ll && !ll_out) return fail_msg(1, who + ": ADX_EM_LOGLIK needs ll_out");
+    return -1;
+}
+
+// The pass with a caller-given workspace (see adx_em_pass).  Without a plan
+// that fits, the statistics fall back to atomics if allow_atomics, else fail.
+static int em_pass_run(const char* name, int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
+                       const void* P, int64_t ldp, const void* Q, int64_t ldq, void* A, void* B, void* T, int flags,
+                       double* ll_out, void* workspace, int64_t workspace_bytes, bool allow_atomics, void* stream) {
+    const int bad = em_pass_check(name, dtype, G, ldg, M, N, K, ldp, ldq, A, B, T, flags, ll_out);
+    if (bad >= 0) return bad;
+    const std::string who(name);
+    const bool stats = flags & ADX_EM_STATS;
+    const bool ll = flags & ADX_EM_LOGLIK;
+    hipStream_t st = (hipStream_t)stream;
+    EmPlan plan;
+    const bool planned = workspace && em_plan(dtype, M, N, K, stats, workspace_bytes, plan);
+    if (!planned) {
+        if (ll || !allow_atomics)
+            return fail_msg(1, who + ": ADX_EM_LOGLIK needs a workspace (adx_em_workspace_bytes, or enough "
+                                     "for one SNP band)");
+        // no workspace: T, A, B by device-scope atomics (order varies run to run)
+        launch_any(dtype, true, false, em_grid(dtype, M, N, K), st, G, ldg, 0, M, N, K, P, ldp, Q, ldq, A, B, T,
+                   nullptr, nullptr);
+        ADX_LAUNCH_CHECK(name);
+        return 0;
+    }
+    uint8_t* ws = static_cast<uint8_t*>(workspace);
+    double* llp = reinterpret_cast<double*>(ws);
+    double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
+    void* tpart = stats ? ws + plan.ll_bytes : nullptr;
+    for (int64_t b = 0; b < plan.nbands; ++b) {
+        const int64_t m0 = b * plan.band;
+        const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
+        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.target);
+        launch_any(dtype, stats, ll, gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, T, llp, tpart);
+        if (ll) reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, slots + b);
+        ADX_LAUNCH_CHECK(name);
+    }
+    if (ll) {
+        reduce_f64_kernel<<<1, 256, 0, st>>>(slots, plan.nbands, ll_out);
+        ADX_LAUNCH_CHECK(name);
+    }
+    return 0;
+}
+
+// Fixed workspace of adx_em_pass_noalloc: one copy of this array per GPU,
+// held from the library's load onto that GPU until the process ends.  In host
+// code its name is only a stand-in; the device address comes from
+// hipGetSymbolAddress, once per device (with that device current), and only
+// that address is handed to kernels.  Every call on a GPU shares the GPU's
+// copy; nothing in it needs clearing (every partial and band slot a call
+// reads was written earlier in the same call), and calls that overlap on
+// one GPU race.
+static __device__ __align__(256) uint8_t scratch[ADX_EM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int noalloc_scratch(void** out) {
+    int dev = -1;
+    ADX_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail_msg(1, "adx_em_pass_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return fail(e, "adx_em_pass_noalloc: hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail_msg(1, "adx_em_pass_noalloc: no device address for the scratch array");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace adx
 
 using namespace adx;
@@ -636,49 +733,26 @@ int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
 int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
                 int64_t ldp, const void* Q, int64_t ldq, void* A, void* B, void* T, int flags,
                 double* ll_out, void* workspace, int64_t workspace_bytes, void* stream) {
-    if (M <= 0 || N <= 0) return fail_msg(1, "adx_em_pass: empty genotype matrix");
-    if (K < 1 || K > 64) return fail_msg(1, "adx_em_pass: K must be in [1, 64]");
-    if (ldg % 16 != 0 || ldg < ceil_div(N, 64) * 64)
-        return fail_msg(1, "adx_em_pass: ldg must be a multiple of 16 and >= N rounded up to 64");
-    if (ldp < K || ldq < K) return fail_msg(1, "adx_em_pass: ldp/ldq < K");
-    if (reinterpret_cast<uintptr_t>(G) % 16 != 0) return fail_msg(1, "adx_em_pass: G not 16-byte aligned");
-    if (dtype != ADX_F32 && dtype != ADX_F64) return fail_msg(1, "adx_em_pass: dtype must be ADX_F32 or ADX_F64");
-    const bool stats = flags & ADX_EM_STATS;
-    const bool ll = flags & ADX_EM_LOGLIK;
-    if (!stats && !ll) return 0;
-    if (ll && dtype != ADX_F64) return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ADX_F64");
-    if (stats && (!A || !B || !T)) return fail_msg(1, "adx_em_pass: A, B, T required");
-    if (ll && !ll_out) return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs ll_out");
-    hipStream_t st = (hipStream_t)stream;
+    return em_pass_run("adx_em_pass", dtype, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, T, flags, ll_out, workspace,
+                       workspace_bytes, true, stream);
+}
+
+int adx_em_pass_noalloc(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
+                        int64_t ldp, const void* Q, int64_t ldq, void* A, void* B, void* T, int flags,
+                        double* ll_out, void* stream) {
+    // shape checks and the band plan first: a shape beyond the fixed scratch
+    // is refused before any device call
+    const int bad = em_pass_check("adx_em_pass_noalloc", dtype, G, ldg, M, N, K, ldp, ldq, A, B, T, flags, ll_out);
+    if (bad >= 0) return bad;
     EmPlan plan;
-    const bool planned = workspace && em_plan(dtype, M, N, K, stats, workspace_bytes, plan);
-    if (!planned) {
-        if (ll)
-            return fail_msg(1, "adx_em_pass: ADX_EM_LOGLIK needs a workspace (adx_em_workspace_bytes, or enough "
-                               "for one SNP band)");
-        // no workspace: T, A, B by device-scope atomics (order varies run to run)
-        launch_any(dtype, true, false, em_grid(dtype, M, N, K), st, G, ldg, 0, M, N, K, P, ldp, Q, ldq, A, B, T,
-                   nullptr, nullptr);
-        ADX_LAUNCH_CHECK("adx_em_pass");
-        return 0;
-    }
-    uint8_t* ws = static_cast<uint8_t*>(workspace);
-    double* llp = reinterpret_cast<double*>(ws);
-    double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
-    void* tpart = stats ? ws + plan.ll_bytes : nullptr;
-    for (int64_t b = 0; b < plan.nbands; ++b) {
-        const int64_t m0 = b * plan.band;
-        const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
-        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.target);
-        launch_any(dtype, stats, ll, gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, T, llp, tpart);
-        if (ll) reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, slots + b);
-        ADX_LAUNCH_CHECK("adx_em_pass");
-    }
-    if (ll) {
-        reduce_f64_kernel<<<1, 256, 0, st>>>(slots, plan.nbands, ll_out);
-        ADX_LAUNCH_CHECK("adx_em_pass(reduce)");
-    }
-    return 0;
+    if (!em_plan(dtype, M, N, K, (flags & ADX_EM_STATS) != 0, ADX_EM_NOALLOC_SCRATCH_BYTES, plan))
+        return fail_msg(1, "adx_em_pass_noalloc: this shape needs more than the fixed scratch of "
+                           "ADX_EM_NOALLOC_SCRATCH_BYTES for one SNP band (use adx_em_pass with a workspace)");
+    void* ws = nullptr;
+    const int rc = noalloc_scratch(&ws);
+    if (rc != 0) return rc;
+    return em_pass_run("adx_em_pass_noalloc", dtype, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, T, flags, ll_out, ws,
+                       ADX_EM_NOALLOC_SCRATCH_BYTES, false, stream);
 }
 
 int adx_em_finalize(int acc_dtype, const double* P, const double* Q, void* A, void* B, void* T,

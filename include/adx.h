@@ -136,6 +136,23 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
                 const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
                 int flags, double *ll_out, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference: adx_em_pass for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, contract and results as
+ * adx_em_pass minus the workspace: the partial sums live in a fixed
+ * ADX_EM_NOALLOC_SCRATCH_BYTES array of device memory that the library holds
+ * on each GPU it runs on until the process ends; the pass runs in as many SNP
+ * bands as that needs (deterministic for a given shape).  Call with the GPU
+ * holding G current.  Allocates nothing (no hipMalloc / hipFree, no
+ * allocating library call).  A shape whose single SNP band does not fit
+ * (about N K x 8 bytes plus the band's A / B partial sums) is refused with an
+ * error before any device work.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (concurrent streams or host threads) race. */
+#define ADX_EM_NOALLOC_SCRATCH_BYTES (128LL << 20)
+int adx_em_pass_noalloc(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, int K,
+                        const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
+                        int flags, double *ll_out, void *stream);
+
 /* Mixed-precision form of adx_em_pass (ADX_EM_STATS only, 1 <= K <= 64;
  * K > 16 in 16-wide k blocks as adx_em_pass),
  * FP32 A, B, T (dtype tag ADX_BF16).  Same contract as adx_em_pass with
