@@ -254,6 +254,8 @@ def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_p
             rec["iterations_to_target"] = hit[0]
             rec["seconds_to_target"] = round(t_init_s + hit[1], 3)
         rec["seconds_to_stop"] = round(t_init_s + (checks[-1][2] if checks else 0.0), 3)
+        if checks and checks[-1][0] > 0:
+            rec["ms_per_iteration"] = round(1e3 * checks[-1][2] / checks[-1][0], 3)
         if extra:
             rec.update(extra)
         out[name] = rec
@@ -290,8 +292,16 @@ def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_p
         if hit is not None:
             rec["iterations_to_target"] = hit[0]
             rec["seconds_to_target"] = round(t_init + hit[1], 3)
+        if checks and checks[-1][0] > 0:
+            rec["ms_per_iteration"] = round(1e3 * checks[-1][2] / checks[-1][0], 3)
         out[f"plain_{prec}"] = rec
         print(f"[race] plain_{prec}: {json.dumps(rec)}", flush=True)
+    # L* is the plain FP32 run's converged logL (the metric's definition);
+    # the best converged logL over the raced forms is reported beside it
+    finals = [r["converged_logL"] for r in out.values() if isinstance(r, dict) and "converged_logL" in r]
+    if finals:
+        out["best_converged_logL"] = max(finals)
+        out["L_star_rel_gap_to_best"] = (max(finals) - L_star) / abs(L_star)
     base = out[f"plain_{plain_precision}"].get("seconds_to_target")
     for name in list(out):
         if isinstance(out[name], dict) and out[name].get("seconds_to_target") and base:

@@ -410,8 +410,10 @@ def em_step_cpu(G, P0, Q0, T, P1, Q1, qb, nthreads=1, fast=False):
 
 
 def adamem_cpu(G, P, Q, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_iter=10000,
-               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None):
-    """em_adam.py:82-179 optimize_parameters (CPU Adam-EM)."""
+               check=5, lr_decay=0.5, min_lr=1e-4, patience=3, tol=0.1, history=None, nthreads=1, fast=False):
+    """em_adam.py:82-179 optimize_parameters (CPU Adam-EM).  ``nthreads`` /
+    ``fast``: the EM statistics loop's OpenMP threads and the build with the
+    reference's fast-math flags (timing only; the default is the checker)."""
     G = np.ascontiguousarray(G, dtype=np.uint8)
     P = _f64(P).copy()
     Q = _f64(Q).copy()
@@ -422,13 +424,13 @@ def adamem_cpu(G, P, Q, *, lr=0.005, beta1=0.80, beta2=0.88, reg_adam=1e-8, max_
     t = 0
 
     def em_step():
-        em_step_cpu(G, P, Q, T, P1, Q1, qb)
+        em_step_cpu(G, P, Q, T, P1, Q1, qb, nthreads, fast)
         P[:] = P1
         Q[:] = Q1
 
     def adam_step():
         nonlocal t
-        em_step_cpu(G, P, Q, T, P1, Q1, qb)
+        em_step_cpu(G, P, Q, T, P1, Q1, qb, nthreads, fast)
         t += 1
         lib().oc_adamP_cpu(_p(P), _p(P1), _p(mP), _p(vP), lr_cur[0], beta1, beta2, reg_adam, t, M, K)
         lib().oc_adamQ_cpu(_p(Q), _p(Q1), _p(mQ), _p(vQ), lr_cur[0], beta1, beta2, reg_adam, t, N, K)
