@@ -70,7 +70,8 @@ def initialize_em_gpu(G, seed: int, M: int, N: int, K: int, device_obj, chunk_si
 def initialize_k(G, N: int, M: int, K: int, f, seed: int, power: int, tol_svd: float, max_als: int,
                  tol_als: float, chunk_size: int, device_obj, threads_per_block: int, original: bool = False,
                  init_original: str = "em", em_init_steps: int = 5):
-    """adamixture.py:185-250 -> (P, Q, initial logL)."""
+    """adamixture.py:185-250 -> (P, Q, initial logL); runs its own rank-K
+    randomized SVD (not the K_max one of ``setup``)."""
     dev = _require_gpu(device_obj)
     logl_calc = utils.get_logl_calculator(dev)
     if original and init_original == "em":
@@ -86,10 +87,14 @@ def initialize_k(G, N: int, M: int, K: int, f, seed: int, power: int, tol_svd: f
 def train_k(G, N: int, M: int, K: int, U_max, S_max, V_max, f, seed: int, lr: float, beta1: float,
             beta2: float, reg_adam: float, max_iter: int, check: int, max_als: int, tol_als: float,
             lr_decay: float, min_lr: float, chunk_size: int, patience: int, tol_adam: float, device_obj,
-            threads_per_block: int, original: bool = False, rtol: float = 0.1, Q_hist: int = 3,
-            init_original: str = "als", P_init=None, Q_init=None, em_init_steps: int = 5,
+            threads_per_block: int, original: bool = False, rtol: float = 1e-7, Q_hist: int = 3,
+            init_original: str = "em", em_init_steps: int = 5, P_init=None, Q_init=None,
             precision: str | None = None):
-    """adamixture.py:503-602 (GPU branches) -> (P, Q) device tensors."""
+    """adamixture.py:253-352 (GPU branches; same parameters, order and
+    defaults) -> (P, Q) device tensors.  The rank-K initialisation slices the
+    K_max SVD of ``setup`` (U_max[:, :K], S_max[:K], V_max[:, :K]) as the
+    reference does; ``precision`` (project extension) selects the Adam-EM
+    pass form."""
     dev = _require_gpu(device_obj)
     if P_init is not None and Q_init is not None:
         P = torch.as_tensor(P_init, dtype=torch.float64, device=dev).contiguous()

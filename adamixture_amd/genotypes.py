@@ -62,8 +62,28 @@ class PackedGenotypes:
         return self.data[:, : self.N].cpu().numpy().copy()
 
 
-# Device copies keyed by the source tensor (dropped when the source dies).
-_CACHE: "weakref.WeakKeyDictionary[torch.Tensor, dict]" = weakref.WeakKeyDictionary()
+# Device copies keyed by the identity of the source tensor; an entry is
+# dropped when its source dies.  (A tensor cannot key a WeakKeyDictionary: a
+# second lookup with the same tensor compares keys with ==, which for
+# tensors is elementwise and raises.)
+_CACHE: dict[int, tuple[weakref.ref, dict]] = {}
+
+
+def _per_source(G: torch.Tensor) -> dict:
+    """The cache slot of source tensor ``G`` (same dict for the same object)."""
+    k = id(G)
+    ent = _CACHE.get(k)
+    if ent is not None and ent[0]() is G:
+        return ent[1]
+
+    def drop(r, k=k):
+        cur = _CACHE.get(k)
+        if cur is not None and cur[0] is r:
+            del _CACHE[k]
+
+    per: dict = {}
+    _CACHE[k] = (weakref.ref(G, drop), per)
+    return per
 
 
 def as_packed(G, M: int, device: torch.device | str | None = None) -> PackedGenotypes:
@@ -84,7 +104,7 @@ def as_packed(G, M: int, device: torch.device | str | None = None) -> PackedGeno
     if Mb != (M + 3) // 4:
         raise ValueError(f"packed genotypes have {Mb} byte rows, expected ceil({M}/4)")
     dev = torch.device(device) if device is not None else (G.device if G.is_cuda else torch.device("cuda"))
-    per_src = _CACHE.setdefault(G, {})
+    per_src = _per_source(G)
     key = (G.data_ptr(), str(dev), M, G._version)
     hit = per_src.get(key)
     if hit is not None:

@@ -151,3 +151,20 @@ def test_timing_vcf_writer_round_trips_through_oracle_parser(tmp_path):
     got, N, M = O.vcf_to_unpacked((tmp_path / "x.vcf").read_bytes())
     assert (N, M) == (37, 1500)
     np.testing.assert_array_equal(got, G)
+
+
+def test_packed_cache_is_keyed_by_tensor_identity():
+    """Repeated setup calls with the same host matrix (the reference's
+    multi-init loop, src/main.py:112-157) reuse one cache slot; another
+    tensor gets its own; a dead source's slot is dropped."""
+    import gc
+    from adamixture_amd import genotypes as GT
+    a = torch.zeros((3, 5), dtype=torch.uint8)
+    b = torch.zeros((3, 5), dtype=torch.uint8)
+    sa = GT._per_source(a)
+    sa["x"] = 1
+    assert GT._per_source(a) is sa and GT._per_source(b) is not sa
+    n = len(GT._CACHE)
+    del a
+    gc.collect()
+    assert len(GT._CACHE) == n - 1
