@@ -43,6 +43,9 @@ SIGNATURES = {
     "adx_em_workspace_bytes": (_i64, [_int, _i64, _i64, _int]),
     "adx_em_pass": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _i64, _p, _i64, _p, _p, _p, _int, _p,
                            _p, _i64, _p]),
+    "adx_em_band_snps": (_i64, [_int, _i64, _i64, _int, _int, _i64, _i64]),
+    "adx_em_pass_host": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _i64, _p, _i64, _p, _p, _p, _int, _p, _p,
+                                _i64, _p, _i64, _p, _p]),
     "adx_em_pass_noalloc": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _i64, _p, _i64, _p, _p, _p, _int,
                                    _p, _p]),
     "adx_em_finalize": (_int, [_int, _p, _p, _p, _p, _p, _p, _i64, _i64, _int, _p, _p, _p]),

@@ -17,7 +17,7 @@ import torch
 from . import _lib
 from .als_gpu import _nn, _tn
 from .em_adam_gpu import em_pass
-from .genotypes import PackedGenotypes, as_packed
+from .genotypes import PackedGenotypes, as_packed, is_streamed
 from .solvers import sqp_solve_p, sqp_solve_q, v_kk
 
 log = logging.getLogger(__name__)
@@ -29,6 +29,20 @@ def grad_hess(G: PackedGenotypes, Q: torch.Tensor, P: torch.Tensor, side: str):
     M, K = P.shape
     N = Q.shape[0]
     rows = N if side == "q" else M
+    if is_streamed(G):   # host-resident genotypes, band by band
+        if side == "q":   # reduction over SNPs: band sums added
+            XtX = torch.zeros((N, K, K), dtype=torch.float64, device=G.device)
+            Xtz = torch.zeros((N, K), dtype=torch.float64, device=G.device)
+            for m0, Mb, band in G.bands():
+                h, z = grad_hess(band, Q, P[m0:m0 + Mb], side)
+                XtX += h
+                Xtz += z
+            return XtX, Xtz
+        XtX = torch.empty((M, K, K), dtype=torch.float64, device=G.device)
+        Xtz = torch.empty((M, K), dtype=torch.float64, device=G.device)
+        for m0, Mb, band in G.bands():   # per SNP: the band's rows
+            XtX[m0:m0 + Mb], Xtz[m0:m0 + Mb] = grad_hess(band, Q, P[m0:m0 + Mb], side)
+        return XtX, Xtz
     XtX = torch.empty((rows, K, K), dtype=torch.float64, device=G.device)
     Xtz = torch.empty((rows, K), dtype=torch.float64, device=G.device)
     _lib.call("adx_grad_hess", 0 if side == "q" else 1, G.data.data_ptr(), G.ldg, G.M, G.N, K,

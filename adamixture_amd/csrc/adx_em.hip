@@ -558,15 +558,19 @@ static int64_t em_plan_bytes(const EmPlan& p) { return p.ll_bytes + p.slice_byte
 // band (then the largest workgroup target, halved down to 1: fewer sample
 // splits, fewer A / B slices) that fits.
 // False if none does.
-static bool em_plan(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t ws, EmPlan& out) {
+// max_band > 0 caps the band (SNPs; the host-streamed pass's staging).
+static bool em_plan(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t ws, EmPlan& out,
+                    int64_t max_band = 0) {
     const EmTiles tl = em_tiles(dtype);
-    out = em_plan_for(dtype, M, N, K, stats, M, tl.target_wgs);
-    if (em_plan_bytes(out) <= ws) return true;
     const int64_t unit = 64 * (em_wide(K) ? 4 : tl.cw);
-    const int64_t cmax = ceil_div(M, unit);
+    out = em_plan_for(dtype, M, N, K, stats, M, tl.target_wgs);
+    if (em_plan_bytes(out) <= ws && (max_band <= 0 || M <= max_band)) return true;
+    int64_t cmax = ceil_div(M, unit);
+    if (max_band > 0 && max_band / unit < cmax) cmax = max_band / unit;
+    if (cmax < 1) return false;
     for (int64_t target = tl.target_wgs; target >= 1; target /= 2) {
         // the slices grow with the band: the widest fitting band by bisection
-        int64_t lo = 0, hi = cmax;   // fits(lo) (or lo = 0), !fits(hi)
+        int64_t lo = 0, hi = cmax + 1;   // fits(lo) (or lo = 0), !fits(hi) (or hi = cmax + 1)
         while (hi - lo > 1) {
             const int64_t c = (lo + hi) / 2;
             if (em_plan_bytes(em_plan_for(dtype, M, N, K, stats, c * unit, target)) <= ws)
@@ -735,6 +739,105 @@ int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, 
                 double* ll_out, void* workspace, int64_t workspace_bytes, void* stream) {
     return em_pass_run("adx_em_pass", dtype, G, ldg, M, N, K, P, ldp, Q, ldq, A, B, T, flags, ll_out, workspace,
                        workspace_bytes, true, stream);
+}
+
+int64_t adx_em_band_snps(int dtype, int64_t M, int64_t N, int K, int flags, int64_t workspace_bytes,
+                          int64_t max_band) {
+    if (M <= 0 || N <= 0 || K < 1 || K > 64 || (dtype != ADX_F32 && dtype != ADX_F64)) return -1;
+    EmPlan plan;
+    if (!em_plan(dtype, M, N, K, (flags & ADX_EM_STATS) != 0, workspace_bytes, plan, max_band)) return -1;
+    return plan.band;
+}
+
+int adx_em_pass_host(int dtype, const uint8_t* G_host, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
+                     int64_t ldp, const void* Q, int64_t ldq, void* A, void* B, void* T, int flags, double* ll_out,
+                     void* workspace, int64_t workspace_bytes, uint8_t* staging, int64_t staging_bytes, void* stream,
+                     void* copy_stream) {
+    const char* name = "adx_em_pass_host";
+    if (!G_host || !staging) return fail_msg(1, "adx_em_pass_host: G_host and staging required");
+    if (reinterpret_cast<uintptr_t>(staging) % 256 != 0) return fail_msg(1, "adx_em_pass_host: staging not 256-byte aligned");
+    const int bad = em_pass_check(name, dtype, staging, ldg, M, N, K, ldp, ldq, A, B, T, flags, ll_out);
+    if (bad >= 0) return bad;
+    const bool stats = flags & ADX_EM_STATS;
+    const bool ll = flags & ADX_EM_LOGLIK;
+    // two staging buffers of whole byte rows, each a multiple of 256 bytes
+    const int64_t half = (staging_bytes / 2) / 256 * 256;
+    const int64_t max_band = (half / ldg) * 4;
+    EmPlan plan;
+    if (!workspace || !em_plan(dtype, M, N, K, stats, workspace_bytes, plan, max_band))
+        return fail_msg(1, "adx_em_pass_host: the workspace and staging do not fit one SNP band "
+                           "(adx_em_band_snps gives the band a workspace allows)");
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t cp = copy_stream ? (hipStream_t)copy_stream : st;
+    uint8_t* buf[2] = {staging, staging + half};
+    hipEvent_t copied[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
+    int rc = 0;
+    for (int i = 0; i < 2 && rc == 0; ++i) {
+        hipError_t e = hipEventCreateWithFlags(&copied[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&used[i], hipEventDisableTiming);
+        if (e != hipSuccess) rc = fail(e, "adx_em_pass_host: hipEventCreateWithFlags");
+    }
+    // the copies start after the work already enqueued on `stream` (which may
+    // still read the staging buffer or write P, Q)
+    if (rc == 0 && cp != st) {
+        hipEvent_t ready = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(ready, st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cp, ready, 0);
+        if (ready) hipEventDestroy(ready);
+        if (e != hipSuccess) rc = fail(e, "adx_em_pass_host: stream fork");
+    }
+    uint8_t* ws = static_cast<uint8_t*>(workspace);
+    double* llp = reinterpret_cast<double*>(ws);
+    double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
+    void* tpart = stats ? ws + plan.ll_bytes : nullptr;
+    for (int64_t b = 0; b < plan.nbands && rc == 0; ++b) {
+        const int64_t m0 = b * plan.band;
+        const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
+        const int64_t rows = (Mb + 3) / 4;
+        const int i = (int)(b & 1);
+        // the copy into buffer i waits for the band that last read it
+        hipError_t e = hipSuccess;
+        if (b >= 2) e = hipStreamWaitEvent(cp, used[i], 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(buf[i], G_host + (m0 >> 2) * ldg, (size_t)(rows * ldg), hipMemcpyHostToDevice, cp);
+        if (e == hipSuccess) e = hipEventRecord(copied[i], cp);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, copied[i], 0);
+        if (e != hipSuccess) {
+            rc = fail(e, "adx_em_pass_host: staging copy");
+            break;
+        }
+        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.target);
+        // the band's rows sit at the start of buffer i: offset G by -m0 / 4 rows
+        // inside launch_band's arithmetic, i.e. pass m0 for P, A, B only
+        const void* Pb = static_cast<const uint8_t*>(P) + m0 * ldp * (dtype == ADX_F64 ? 8 : 4);
+        void* Ab = stats ? static_cast<uint8_t*>(A) + m0 * K * (dtype == ADX_F64 ? 8 : 4) : nullptr;
+        void* Bb = stats ? static_cast<uint8_t*>(B) + m0 * K * (dtype == ADX_F64 ? 8 : 4) : nullptr;
+        launch_any(dtype, stats, ll, gd, st, buf[i], ldg, 0, Mb, N, K, Pb, ldp, Q, ldq, Ab, Bb, T, llp, tpart);
+        if (ll) reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, slots + b);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(used[i], st);
+        if (e != hipSuccess) rc = fail(e, name);
+    }
+    if (rc == 0 && ll) {
+        reduce_f64_kernel<<<1, 256, 0, st>>>(slots, plan.nbands, ll_out);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(e, name);
+    }
+    // the copy stream's last use of the buffers is ordered before later work on st
+    if (rc == 0 && cp != st) {
+        hipEvent_t done = nullptr;
+        hipError_t e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(done, cp);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, done, 0);
+        if (done) hipEventDestroy(done);
+        if (e != hipSuccess) rc = fail(e, "adx_em_pass_host: stream join");
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (copied[k]) hipEventDestroy(copied[k]);
+        if (used[k]) hipEventDestroy(used[k]);
+    }
+    return rc;
 }
 
 int adx_em_pass_noalloc(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,

@@ -23,7 +23,7 @@ import time
 import torch
 
 from . import _lib
-from .genotypes import PackedGenotypes, as_packed
+from .genotypes import PackedGenotypes, as_packed, is_streamed
 
 log = logging.getLogger(__name__)
 
@@ -59,8 +59,8 @@ class _Workspace:
         self.min_cap = 256 << 20
         self.frac = 0.1
 
-    def cap(self, G: PackedGenotypes) -> int:
-        return max(int(self.frac * G.data.numel()), self.min_cap)
+    def cap(self, G) -> int:
+        return max(int(self.frac * G.host.numel() if is_streamed(G) else self.frac * G.data.numel()), self.min_cap)
 
     def shrink(self) -> bool:
         """Halve the cap after an out-of-memory error (False at the floor)."""
@@ -92,10 +92,30 @@ def em_pass(G: PackedGenotypes, P, Q, A=None, B=None, T=None, *, dtype: int = _l
     # K = 0 sizes only the log-likelihood partials (no T slices without stats)
     ws_bytes = min(_lib.load().adx_em_workspace_bytes(dtype, G.M, G.N, K if stats else 0), _WS.cap(G))
     ws, ll = _WS.get(G.device, ws_bytes)
-    _lib.call("adx_em_pass", dtype, G.data.data_ptr(), G.ldg, G.M, G.N, K, P.data_ptr(), P.stride(0),
-              Q.data_ptr(), Q.stride(0), _lib.ptr(A), _lib.ptr(B), _lib.ptr(T), flags,
-              _lib.ptr(ll) if loglik else None, ws.data_ptr(), ws_bytes, _lib.stream_handle(G.device))
+    head = (dtype,)
+    tail = (G.M, G.N, K, P.data_ptr(), P.stride(0), Q.data_ptr(), Q.stride(0), _lib.ptr(A), _lib.ptr(B),
+            _lib.ptr(T), flags, _lib.ptr(ll) if loglik else None, ws.data_ptr(), ws_bytes)
+    if is_streamed(G):
+        _em_pass_streamed(G, head + (G.host.data_ptr(), G.ldg) + tail, dtype, K, flags, ws_bytes)
+    else:
+        _lib.call("adx_em_pass", *head, G.data.data_ptr(), G.ldg, *tail, _lib.stream_handle(G.device))
     return ll if loglik else None
+
+
+def _em_pass_streamed(G, args, dtype: int, K: int, flags: int, ws_bytes: int) -> None:
+    """adx_em_pass_host on a host-resident matrix: the staging holds the
+    plan's band twice when G.staging_cap allows (then the result is
+    bit-identical to the resident pass with the same workspace), else the
+    bands shrink to what it holds."""
+    import torch as _t
+    band = _lib.load().adx_em_band_snps(dtype, G.M, G.N, K, flags, ws_bytes, 0)
+    half = -(-((band + 3) // 4) * G.ldg // 256) * 256 if band > 0 else 0
+    want = min(2 * half, max(G.staging_cap, 2 * 256 * ((G.ldg + 255) // 256)))
+    buf = G.staging(want)
+    if G.copy_stream is None:
+        G.copy_stream = _t.cuda.Stream(device=G.device)
+    _lib.call("adx_em_pass_host", *args, buf.data_ptr(), buf.numel(), _lib.stream_handle(G.device),
+              G.copy_stream.cuda_stream)
 
 
 class EMAdamOptimizer:
@@ -133,6 +153,9 @@ class EMAdamOptimizer:
         if self.precision == "f64":
             return em_pass(G, P, Q, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F64, loglik=loglik)
         if self.precision == "bf16":
+            if is_streamed(G):
+                raise NotImplementedError("the mixed-precision (bf16) pass needs the genotypes resident on the GPU; "
+                                          "host-streamed genotypes run the f64 or f32 form")
             st = _lib.stream_handle(self.device)
             M, K = P.shape
             N = Q.shape[0]

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .genotypes import PackedGenotypes, as_packed
+from .genotypes import PackedGenotypes, as_packed, is_streamed
 
 log = logging.getLogger(__name__)
 
@@ -58,6 +58,11 @@ def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
     L = W.shape[1]
     if L > MAX_L:   # column blocks: each column of Y depends on one column of W only
         return torch.cat([xw(G, f, W[:, c:c + MAX_L].contiguous()) for c in range(0, L, MAX_L)], dim=1)
+    if is_streamed(G):   # host-resident genotypes: rows of Y band by band
+        Y = torch.empty((G.M, L), dtype=torch.float32, device=G.device)
+        for m0, Mb, band in G.bands():
+            Y[m0:m0 + Mb] = xw(band, f[m0:m0 + Mb], W)
+        return Y
     Y = torch.zeros((G.M, L), dtype=torch.float32, device=G.device)
     _lib.call("adx_svd_xw", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.contiguous().data_ptr(), L,
               Y.data_ptr(), _lib.ptr(_work(G, 0, L)), _lib.stream_handle(G.device))
@@ -69,6 +74,11 @@ def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
     L = Y.shape[1]
     if L > MAX_L:
         return torch.cat([xty(G, f, Y[:, c:c + MAX_L].contiguous()) for c in range(0, L, MAX_L)], dim=1)
+    if is_streamed(G):   # host-resident genotypes: the SNP reduction summed band by band
+        A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
+        for m0, Mb, band in G.bands():
+            A += xty(band, f[m0:m0 + Mb], Y[m0:m0 + Mb])
+        return A
     A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
     _lib.call("adx_svd_xty", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.contiguous().data_ptr(), L,
               A.data_ptr(), _lib.ptr(_work(G, 1, L)), _lib.stream_handle(G.device))

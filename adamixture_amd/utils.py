@@ -15,7 +15,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .genotypes import PackedGenotypes, allele_frequencies, as_packed, unpack_center_chunk, unpack_chunk
+from .genotypes import (HostPackedGenotypes, PackedGenotypes, allele_frequencies, as_packed, unpack_center_chunk,
+                        unpack_chunk)
 
 log = logging.getLogger(__name__)
 
@@ -132,26 +133,36 @@ def estimate_gpu_bytes(M: int, N: int, K: int, algorithm: str, q_hist: int = 3,
     return max(svd, als, opt) if include_initialization else opt
 
 
+# Device bytes the host-streamed path stages genotype rows in (two halves of
+# the EM pass's band, or one band of the other passes).
+STREAM_STAGING_BYTES = 1 << 30
+
+
 def manage_gpu_memory(G, device: torch.device, M: int, N: int, K: int, chunk_size: int, algorithm: str,
                       q_hist: int = 3, include_initialization: bool = True):
-    """utils.py:311-429: place G on the device when it fits.
-
-    Returns PackedGenotypes (device-resident, padded layout).  If G and the
-    working buffers do not fit in 95% of free memory a RuntimeError naming
-    "out of memory" is raised (host-streamed EM is not implemented)."""
+    """utils.py:311-429: place G on the device when it and the working
+    buffers fit 95% of free memory (PackedGenotypes, padded layout), else
+    keep it in pinned host memory and stream its byte rows through a device
+    staging buffer (HostPackedGenotypes; the reference keeps G on the CPU
+    then, :425-428).  If not even the buffers and the staging fit, a
+    RuntimeError naming "out of memory" is raised."""
     dev = torch.device(device)
     if dev.type != "cuda":
         return G
-    if isinstance(G, PackedGenotypes):
+    if isinstance(G, (PackedGenotypes, HostPackedGenotypes)):
         return G
-    need = ((M + 3) // 4) * ((N + 63) // 64 * 64) + estimate_gpu_bytes(M, N, K, algorithm, q_hist,
-                                                                      include_initialization)
-    free_mb = get_free_gpu_memory(dev)
-    if free_mb * 0.95 * 2**20 < need:
-        raise RuntimeError(f"out of memory: genotypes and buffers need {need / 2**30:.1f} GiB, "
-                           f"{free_mb / 1024:.1f} GiB free")
-    log.info("    Moving genotype matrix to GPU...")
-    return as_packed(G, M, dev)
+    g_bytes = ((M + 3) // 4) * ((N + 63) // 64 * 64)
+    buffers = estimate_gpu_bytes(M, N, K, algorithm, q_hist, include_initialization)
+    avail = get_free_gpu_memory(dev) * 0.95 * 2**20
+    if g_bytes + buffers <= avail:
+        log.info("    Moving genotype matrix to GPU...")
+        return as_packed(G, M, dev)
+    staging = min(STREAM_STAGING_BYTES, max(avail - buffers, 0))
+    if staging < 4 * ((N + 63) // 64 * 64):
+        raise RuntimeError(f"out of memory: genotypes ({g_bytes / 2**30:.1f} GiB) do not fit and the buffers need "
+                           f"{buffers / 2**30:.1f} GiB of {avail / 2**30:.1f} GiB free")
+    log.info("    Genotype matrix too large for GPU, keeping it on the host (streamed in SNP bands)...")
+    return HostPackedGenotypes.from_packed(G, M, dev, staging_cap=int(staging))
 
 
 def write_outputs(Q: np.ndarray, run_name: str, K: int, out_path, P: np.ndarray | None = None) -> None:

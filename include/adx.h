@@ -136,6 +136,30 @@ int adx_em_pass(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int64_t N, 
                 const void *P, int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T,
                 int flags, double *ll_out, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* The SNP band (in SNPs) adx_em_pass runs with for this shape and workspace
+ * (M if one band; max_band > 0 caps it), or -1 if the workspace does not fit
+ * one band. */
+int64_t adx_em_band_snps(int dtype, int64_t M, int64_t N, int K, int flags, int64_t workspace_bytes,
+                         int64_t max_band);
+
+/* Host-streamed form of adx_em_pass for genotype matrices kept in host memory
+ * because they do not fit the GPU (the reference keeps G on the host then and
+ * copies each byte-row chunk per use, adamixture/src/utils.py:425-428,
+ * :538-557).  G_host is the padded layout (ldg, 0xFF padding) in host memory
+ * (pinned for asynchronous copies).  The pass runs the SNP bands of
+ * adx_em_pass's plan for workspace_bytes, capped to what half the staging
+ * buffer (device, 256-byte aligned) holds: each band's byte rows are copied
+ * into one of two staging halves on copy_stream (NULL: on stream), ordered
+ * by events against the band that last read that half, then processed as
+ * adx_em_pass processes a resident band.  With the same workspace and a
+ * staging that holds the plan's band twice, A, B, T and the log-likelihood
+ * are bit-identical to adx_em_pass on the resident matrix.  Returns after
+ * enqueueing; later work on `stream` is ordered after every copy. */
+int adx_em_pass_host(int dtype, const uint8_t *G_host, int64_t ldg, int64_t M, int64_t N, int K, const void *P,
+                     int64_t ldp, const void *Q, int64_t ldq, void *A, void *B, void *T, int flags,
+                     double *ll_out, void *workspace, int64_t workspace_bytes, uint8_t *staging,
+                     int64_t staging_bytes, void *stream, void *copy_stream);
+
 /* No counterpart in the reference: adx_em_pass for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, contract and results as

@@ -1,5 +1,6 @@
 """Host-side logic of the drop-in layer that needs no GPU: the memory
-manager's placement decision with a faked free-memory value (the reference's
+manager's placement decision (device, host-streamed, or out of memory) with
+a faked free-memory value (the reference's
 own approach to testing it, adamixture/src/utils.py:311-429), the OOM
 classifier and chunk halving (utils.py:19-89), and the .P / .Q writer
 format (utils.py:252-273)."""
@@ -19,7 +20,16 @@ def test_manage_gpu_memory_places_or_raises(monkeypatch):
     monkeypatch.setattr(utils, "get_free_gpu_memory", lambda dev: need / 2**20 / 0.95 * 1.01)
     assert utils.manage_gpu_memory(G, torch.device("cuda:0"), M, N, K, 8192, "adamem") == "device-copy"
     assert placed == [(M, "cuda:0")]
+    # G does not fit, the buffers and a staging buffer do: host-streamed
+    streamed = []
+    monkeypatch.setattr(utils.HostPackedGenotypes, "from_packed",
+                        classmethod(lambda cls, g, m, dev, staging_cap: streamed.append((m, staging_cap)) or "host"))
     monkeypatch.setattr(utils, "get_free_gpu_memory", lambda dev: need / 2**20 / 0.95 * 0.99)
+    assert utils.manage_gpu_memory(G, torch.device("cuda:0"), M, N, K, 8192, "adamem") == "host"
+    assert streamed == [(M, utils.STREAM_STAGING_BYTES)]
+    # not even the buffers fit: out of memory, as the reference classifies it
+    buffers = utils.estimate_gpu_bytes(M, N, K, "adamem")
+    monkeypatch.setattr(utils, "get_free_gpu_memory", lambda dev: buffers / 2**20 / 0.95 * 0.99)
     with pytest.raises(RuntimeError, match="out of memory") as ei:
         utils.manage_gpu_memory(G, torch.device("cuda:0"), M, N, K, 8192, "adamem")
     assert utils.is_gpu_oom(ei.value)
