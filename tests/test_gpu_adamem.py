@@ -105,3 +105,28 @@ def test_cpu_ref_workload_fixed_iterations(cuda, precision):
     Gl = G
     assert loglikelihood_gpu(Gl, torch.from_numpy(Po).to(cuda), torch.from_numpy(Qo).to(cuda), w.M, w.N) == \
         pytest.approx(O.loglik_gpu(G.to_host_packed(), Po, Qo, w.M), rel=1e-12)
+
+
+def test_cpu_ref_workload_as_configured_within_1e5_relative(cuda):
+    """The 1,000 x 20,000, K=3 workload as configured: randomized SVD + ALS
+    init (seed 42, through the drop-in driver's initialize_k), then 100
+    Adam-EM iterations with the reference defaults, FP64 form, on the GPU
+    and on the oracle from that same init.  P, Q (elementwise) and logL
+    (every check) within 1e-5 relative, the same check iterations."""
+    from adamixture_amd.adamixture import initialize_k, setup
+    from adamixture_amd.synth import WORKLOADS, draw_params, make_genotypes
+    w = WORKLOADS["cpu_ref"]
+    G, _, _ = make_genotypes(w, cuda, params=draw_params(w))
+    dev, tpb, f, _, _, _, G = setup(G, w.N, w.M, w.K, w.seed, 5, 0.1, 8192, cuda, compute_svd=False)
+    P0, Q0, _ = initialize_k(G, w.N, w.M, w.K, f, w.seed, 5, 0.1, 1000, 1e-4, 8192, dev, tpb)
+    P0, Q0 = P0.cpu().numpy(), Q0.cpu().numpy()
+    P, Q, hist = _run(G, P0, Q0, w.M, w.N, w.K, cuda, "f64", max_iter=100)
+    ohist = []
+    Po, Qo = O.adamem_gpu(G.to_host_packed(), P0, Q0, w.M, max_iter=100, history=ohist)
+    rP = np.abs(P - Po) / np.abs(Po)
+    rQ = np.abs(Q - Qo) / np.abs(Qo)
+    rL = np.abs(np.array([L for _, L in hist]) - np.array([L for _, L in ohist])) / np.abs([L for _, L in ohist])
+    print(f"C1 as configured: checks {len(hist)}, max rel dP {rP.max():.3e} (median {np.median(rP):.1e}), "
+          f"max rel dQ {rQ.max():.3e}, max rel dlogL {rL.max():.3e}")
+    assert [it for it, _ in hist] == [it for it, _ in ohist]
+    assert rP.max() <= 1e-5 and rQ.max() <= 1e-5 and rL.max() <= 1e-5
