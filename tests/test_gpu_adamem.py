@@ -130,3 +130,41 @@ def test_cpu_ref_workload_as_configured_within_1e5_relative(cuda):
           f"max rel dQ {rQ.max():.3e}, max rel dlogL {rL.max():.3e}")
     assert [it for it, _ in hist] == [it for it, _ in ohist]
     assert rP.max() <= 1e-5 and rQ.max() <= 1e-5 and rL.max() <= 1e-5
+
+
+def test_gpu_f32_workload_as_configured(cuda):
+    """The 10,000 x 100,000, K=8 workload (Balding-Nichols P, Dirichlet(0.3)
+    Q, 0.5% missing) as configured: bit-exact unpack against the oracle's
+    generator, SVD + ALS init through the driver, then 20 Adam-EM iterations
+    from that init in the FP64 form (P, Q, logL within 1e-5 relative of the
+    oracle) and in the FP32 form (logL within 1e-6 relative; P and Q within
+    2 lr: a parameter at its fixed point steps +-lr per Adam step, and the
+    FP32 pass's rounding can flip the sign of its tiny EM displacement)."""
+    from adamixture_amd import genotypes as GTm
+    from adamixture_amd.adamixture import initialize_k, setup
+    from adamixture_amd.synth import WORKLOADS, draw_params, make_genotypes
+    w = WORKLOADS["gpu_f32"]
+    Ptrue, Qtrue = draw_params(w)
+    G, _, _ = make_genotypes(w, cuda, params=(Ptrue, Qtrue))
+    Gp = G.to_host_packed()
+    rows = slice(1000, 1250)
+    assert np.array_equal(Gp[rows], O.synth_packed(Ptrue, Qtrue, w.seed, w.miss, row0=1000, rows=250))
+    assert np.array_equal(GTm.unpack_chunk(G, 4000, 1000).cpu().numpy(), O.unpack(Gp, w.M, 4000, 1000))
+    dev, tpb, f, _, _, _, G = setup(G, w.N, w.M, w.K, w.seed, 5, 0.1, 8192, cuda, compute_svd=False)
+    P0, Q0, _ = initialize_k(G, w.N, w.M, w.K, f, w.seed, 5, 0.1, 1000, 1e-4, 8192, dev, tpb)
+    P0, Q0 = P0.cpu().numpy(), Q0.cpu().numpy()
+    ohist = []
+    Po, Qo = O.adamem_gpu(Gp, P0, Q0, w.M, max_iter=20, history=ohist)
+    Lo = np.array([L for _, L in ohist])
+    for precision in ("f64", "f32"):
+        P, Q, hist = _run(G, P0, Q0, w.M, w.N, w.K, cuda, precision, max_iter=20)
+        L = np.array([L for _, L in hist])
+        assert [it for it, _ in hist] == [it for it, _ in ohist]
+        rP, rQ = np.abs(P - Po) / np.abs(Po), np.abs(Q - Qo) / np.abs(Qo)
+        rL = np.abs(L - Lo) / np.abs(Lo)
+        print(f"C2 {precision}: max rel dP {rP.max():.3e}, max |dP| {np.abs(P - Po).max():.3e}, max rel dQ "
+              f"{rQ.max():.3e}, max rel dlogL {rL.max():.3e}")
+        if precision == "f64":
+            assert rP.max() <= 1e-5 and rQ.max() <= 1e-5 and rL.max() <= 1e-5
+        else:
+            assert rL.max() <= 1e-6 and np.abs(P - Po).max() <= 2 * HP["lr"] and np.abs(Q - Qo).max() <= 2 * HP["lr"]
