@@ -57,6 +57,9 @@ SIGNATURES = {
     "adx_svd_xw": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p, _p]),
     "adx_svd_xty": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p, _p]),
     "adx_svd_work_floats": (_i64, [_int, _i64, _i64, _int]),
+    "adx_svd_xw_h2": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p, _i64, _p]),
+    "adx_svd_xty_h2": (_int, [_p, _i64, _i64, _i64, _p, _p, _int, _p, _p, _i64, _p]),
+    "adx_svd_h2_work_bytes": (_i64, [_int, _i64, _i64, _int]),
     "adx_grad_hess": (_int, [_int, _p, _i64, _i64, _i64, _int, _p, _p, _p, _p, _p]),
     "adx_sqp_solve": (_int, [_int, _p, _p, _p, _p, _p, _i64, _int, _p, _i64, _p]),
     "adx_bvls_workspace_bytes": (_i64, [_int]),

@@ -1,9 +1,14 @@
 """Randomized SVD initializer on the GPU (drop-in for adamixture/model/svd_gpu.py).
 
-The streamed GEMMs with the centred genotype matrix run in libadx
-(adx_svd_xw / adx_svd_xty, decode fused, FP32 MFMA); the L x L
-eigen-decompositions (L = max(k + 10, 20)) run on the host in FP32 LAPACK,
-as the reference CPU path does (svd.py:13-28).
+The streamed GEMMs with the centred genotype matrix run in libadx, decode
+fused, in one of two arithmetics (``PRODUCTS``):
+  * "h2" (default): adx_svd_xw_h2 / adx_svd_xty_h2, FP16 MFMA on two-term
+    FP16 splits of the operands with FP32 accumulation -- the FP32 forms'
+    accuracy at a quarter of the matrix-core time;
+  * "f32": adx_svd_xw / adx_svd_xty, FP32 MFMA, the reference's SGEMM
+    arithmetic (svd_gpu.py:104-107, :133-138).
+The L x L eigen-decompositions (L = max(k + 10, 20)) run on the host in
+FP32 LAPACK, as the reference CPU path does (svd.py:13-28).
 
 The Gaussian test matrix is drawn on the host with
 ``numpy.random.default_rng(seed).standard_normal((M, L), float32)`` exactly
@@ -33,19 +38,30 @@ def eigSVD_host(C: np.ndarray):
     return (np.ascontiguousarray(U[:, ::-1]), np.ascontiguousarray(S[::-1]), np.ascontiguousarray(V[:, ::-1]))
 
 
+# Product arithmetic of xw / xty: "h2" (FP16 two-term splits on FP16 MFMA)
+# or "f32" (FP32 MFMA).
+PRODUCTS = "h2"
+
 _WORK: dict = {}
 
 
-def _work(G: PackedGenotypes, which: int, L: int):
-    """Workspace for the fixed-order split reduction (None if the grid does
-    not split the reduction)."""
-    n = _lib.load().adx_svd_work_floats(which, G.M, G.N, L)
-    if n <= 0:
+def _work(G: PackedGenotypes, nbytes: int):
+    """Per-device scratch of at least nbytes (None for 0); the caching
+    allocator returns 256-byte aligned blocks."""
+    if nbytes <= 0:
         return None
     w = _WORK.get(G.device)
-    if w is None or w.numel() < n:
-        w = _WORK[G.device] = torch.empty(n, dtype=torch.float32, device=G.device)
+    if w is None or w.numel() < nbytes:
+        _WORK[G.device] = None
+        w = _WORK[G.device] = torch.empty(nbytes, dtype=torch.uint8, device=G.device)
     return w
+
+
+def _products(products):
+    p = PRODUCTS if products is None else products
+    if p not in ("h2", "f32"):
+        raise ValueError(f"unknown SVD product arithmetic {p!r} (h2 or f32)")
+    return p
 
 
 # Columns per decode-fused GEMM launch (the kernels take L <= 64; the
@@ -53,35 +69,53 @@ def _work(G: PackedGenotypes, which: int, L: int):
 MAX_L = 64
 
 
-def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+def xw(G: PackedGenotypes, f: torch.Tensor, W: torch.Tensor, products: str | None = None) -> torch.Tensor:
     """Y = X W (M x L) with X the centred genotypes, W (N x L) FP32."""
+    p = _products(products)
     L = W.shape[1]
     if L > MAX_L:   # column blocks: each column of Y depends on one column of W only
-        return torch.cat([xw(G, f, W[:, c:c + MAX_L].contiguous()) for c in range(0, L, MAX_L)], dim=1)
+        return torch.cat([xw(G, f, W[:, c:c + MAX_L].contiguous(), p) for c in range(0, L, MAX_L)], dim=1)
     if is_streamed(G):   # host-resident genotypes: rows of Y band by band
         Y = torch.empty((G.M, L), dtype=torch.float32, device=G.device)
         for m0, Mb, band in G.bands():
-            Y[m0:m0 + Mb] = xw(band, f[m0:m0 + Mb], W)
+            Y[m0:m0 + Mb] = xw(band, f[m0:m0 + Mb], W, p)
+        return Y
+    W = W.contiguous()
+    st = _lib.stream_handle(G.device)
+    if p == "h2":
+        nb = _lib.load().adx_svd_h2_work_bytes(0, G.M, G.N, L)
+        Y = torch.empty((G.M, L), dtype=torch.float32, device=G.device)
+        _lib.call("adx_svd_xw_h2", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.data_ptr(), L,
+                  Y.data_ptr(), _lib.ptr(_work(G, nb)), nb, st)
         return Y
     Y = torch.zeros((G.M, L), dtype=torch.float32, device=G.device)
-    _lib.call("adx_svd_xw", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.contiguous().data_ptr(), L,
-              Y.data_ptr(), _lib.ptr(_work(G, 0, L)), _lib.stream_handle(G.device))
+    _lib.call("adx_svd_xw", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), W.data_ptr(), L,
+              Y.data_ptr(), _lib.ptr(_work(G, 4 * _lib.load().adx_svd_work_floats(0, G.M, G.N, L))), st)
     return Y
 
 
-def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor, products: str | None = None) -> torch.Tensor:
     """A = X^T Y (N x L) with Y (M x L) FP32."""
+    p = _products(products)
     L = Y.shape[1]
     if L > MAX_L:
-        return torch.cat([xty(G, f, Y[:, c:c + MAX_L].contiguous()) for c in range(0, L, MAX_L)], dim=1)
+        return torch.cat([xty(G, f, Y[:, c:c + MAX_L].contiguous(), p) for c in range(0, L, MAX_L)], dim=1)
     if is_streamed(G):   # host-resident genotypes: the SNP reduction summed band by band
         A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
         for m0, Mb, band in G.bands():
-            A += xty(band, f[m0:m0 + Mb], Y[m0:m0 + Mb])
+            A += xty(band, f[m0:m0 + Mb], Y[m0:m0 + Mb], p)
+        return A
+    Y = Y.contiguous()
+    st = _lib.stream_handle(G.device)
+    if p == "h2":
+        nb = _lib.load().adx_svd_h2_work_bytes(1, G.M, G.N, L)
+        A = torch.empty((G.N, L), dtype=torch.float32, device=G.device)
+        _lib.call("adx_svd_xty_h2", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.data_ptr(), L,
+                  A.data_ptr(), _lib.ptr(_work(G, nb)), nb, st)
         return A
     A = torch.zeros((G.N, L), dtype=torch.float32, device=G.device)
-    _lib.call("adx_svd_xty", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.contiguous().data_ptr(), L,
-              A.data_ptr(), _lib.ptr(_work(G, 1, L)), _lib.stream_handle(G.device))
+    _lib.call("adx_svd_xty", G.data.data_ptr(), G.ldg, G.M, G.N, f.data_ptr(), Y.data_ptr(), L,
+              A.data_ptr(), _lib.ptr(_work(G, 4 * _lib.load().adx_svd_work_floats(1, G.M, G.N, L))), st)
     return A
 
 

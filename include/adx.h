@@ -261,6 +261,21 @@ int adx_svd_xty(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float
                 float *A, float *work, void *stream);
 int64_t adx_svd_work_floats(int which, int64_t M, int64_t N, int L);
 
+/* The same products in FP16 arithmetic on two-term operand splits (16x16x32
+ * FP16 MFMA, FP32 accumulation), accurate to FP32 rounding like the FP32
+ * forms: xw splits x = g - 2f per SNP and W s (s a power of two), xty
+ * computes Y^T Gm - (2f Y)^T Mask on splits of Y s and 2f Y s.  They replace
+ * the same reference SGEMMs as adx_svd_xw / adx_svd_xty.  The output is
+ * overwritten (no zeroing needed).  work: at least adx_svd_h2_work_bytes(0
+ * for xw / 1 for xty, M, N, L) bytes, 256-byte aligned, holding the split
+ * operand, its scale and the split-reduction slices (deterministic).
+ * 1 <= L <= 64. */
+int adx_svd_xw_h2(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float *f, const float *W, int L,
+                  float *Y, void *work, int64_t work_bytes, void *stream);
+int adx_svd_xty_h2(const uint8_t *G, int64_t ldg, int64_t M, int64_t N, const float *f, const float *Y, int L,
+                   float *A, void *work, int64_t work_bytes, void *stream);
+int64_t adx_svd_h2_work_bytes(int which, int64_t M, int64_t N, int L);
+
 /* --- SQP block-relaxation gradient / Hessian (FP64, 1 <= K <= 64) ---------- */
 
 /* Replaces compute_grad_hess_Q_gpu (side 0) and compute_grad_hess_P_gpu

@@ -50,6 +50,50 @@ def test_svd_gemms_match_dense(cuda):
     np.testing.assert_allclose(got_a, X.T @ Y, rtol=1e-4, atol=1e-3)
 
 
+def _col_err(got, want):
+    """Largest error of a column relative to that column's largest entry."""
+    return float((np.abs(got - want).max(0) / np.maximum(np.abs(want).max(0), 1e-300)).max())
+
+
+@pytest.mark.parametrize("M,N,L,scale", [(1037, 411, 20, 1.0), (20003, 3001, 1, 1.0), (20003, 3001, 33, 1.0),
+                                         (4001, 2049, 64, 1.0), (4001, 2049, 20, 1e-6), (4001, 2049, 20, 3e5),
+                                         (5, 17, 20, 1.0)])
+def test_svd_products_h2_match_fp32_accuracy(cuda, M, N, L, scale):
+    """The FP16 two-term forms against the FP64 dense products: within 2e-6 of
+    each column's largest entry, and not more than twice the FP32 forms'
+    error (plus 2e-7) -- ragged M and N, L = 1 .. 64, one and several
+    reduction splits, operands scaled far from 1 (the power-of-two scale)."""
+    rng = np.random.default_rng(M + L)
+    P = rng.uniform(0.05, 0.95, size=(M, 3))
+    Q = rng.dirichlet(np.ones(3), size=N)
+    Gp = O.synth_packed(P, Q, 3, 0.03)
+    G = as_packed(torch.from_numpy(Gp), M, cuda)
+    f = O.freq_gpu(Gp, M)
+    X = O.unpack_center(Gp, f, M, 0, M).astype(np.float64)
+    W = (rng.normal(size=(N, L)) * scale).astype(np.float32)
+    Y = (rng.normal(size=(M, L)) * scale).astype(np.float32)
+    fd = torch.from_numpy(f).to(cuda)
+    Wd, Yd = torch.from_numpy(W).to(cuda), torch.from_numpy(Y).to(cuda)
+    want_y, want_a = X @ W.astype(np.float64), X.T @ Y.astype(np.float64)
+    print(f"case: h2 products M={M} N={N} L={L} scale={scale}", flush=True)
+    for prod, fn, Z, want in (("xw", xw, Wd, want_y), ("xty", xty, Yd, want_a)):
+        e_h2 = _col_err(fn(G, fd, Z, "h2").cpu().numpy(), want)
+        e_32 = _col_err(fn(G, fd, Z, "f32").cpu().numpy(), want)
+        print(f"{prod}: h2 {e_h2:.2e}  f32 {e_32:.2e}", flush=True)
+        assert e_h2 <= 2e-6 and e_h2 <= 2 * e_32 + 2e-7, (prod, e_h2, e_32)
+
+
+def test_svd_products_h2_zero_operand(cuda):
+    """An all-zero operand (scale 1) gives exact zeros."""
+    rng = np.random.default_rng(9)
+    M, N, L = 999, 300, 20
+    Gp = O.synth_packed(rng.uniform(0.05, 0.95, size=(M, 2)), rng.dirichlet(np.ones(2), size=N), 1, 0.02)
+    G = as_packed(torch.from_numpy(Gp), M, cuda)
+    fd = torch.from_numpy(O.freq_gpu(Gp, M)).to(cuda)
+    assert not xw(G, fd, torch.zeros(N, L, device=cuda), "h2").any()
+    assert not xty(G, fd, torch.zeros(M, L, device=cuda), "h2").any()
+
+
 def test_init_products_are_deterministic(cuda):
     """The split reductions of the SVD GEMMs (several splits at this shape)
     and the ALS X^T Y products are summed in a fixed order: repeated calls
@@ -66,7 +110,8 @@ def test_init_products_are_deterministic(cuda):
     fd = torch.from_numpy(O.freq_gpu(Gp, M)).to(cuda)
     W = torch.from_numpy(rng.normal(size=(N, L)).astype(np.float32)).to(cuda)
     Y = torch.from_numpy(rng.normal(size=(M, L)).astype(np.float32)).to(cuda)
-    assert torch.equal(xw(G, fd, W), xw(G, fd, W)) and torch.equal(xty(G, fd, Y), xty(G, fd, Y))
+    for p in ("h2", "f32"):
+        assert torch.equal(xw(G, fd, W, p), xw(G, fd, W, p)) and torch.equal(xty(G, fd, Y, p), xty(G, fd, Y, p))
     Xd = torch.from_numpy(rng.normal(size=(300000, 10))).to(cuda)
     Yd = torch.from_numpy(rng.normal(size=(300000, 12))).to(cuda)
     C1, C2 = _tn(Xd, Yd), _tn(Xd, Yd)
