@@ -7,8 +7,9 @@ fused, in one of two arithmetics (``PRODUCTS``):
     accuracy at a quarter of the matrix-core time;
   * "f32": adx_svd_xw / adx_svd_xty, FP32 MFMA, the reference's SGEMM
     arithmetic (svd_gpu.py:104-107, :133-138).
-The L x L eigen-decompositions (L = max(k + 10, 20)) run on the host in
-FP32 LAPACK, as the reference CPU path does (svd.py:13-28).
+The eigSVD steps keep the tall matrices on the device: the L x L Gram
+matrix and U = C V S^-1 are FP64 device products and only the L x L
+eigen-decomposition (L = max(k + 10, 20)) runs on the host in LAPACK.
 
 The Gaussian test matrix is drawn on the host with
 ``numpy.random.default_rng(seed).standard_normal((M, L), float32)`` exactly
@@ -30,17 +31,42 @@ from .genotypes import PackedGenotypes, as_packed, is_streamed
 log = logging.getLogger(__name__)
 
 
-def eigSVD_host(C: np.ndarray):
-    """svd.py:13-28 eigSVD of C (n x L): eigh of C^T C, descending order."""
-    D, V = np.linalg.eigh(C.T @ C)
-    S = np.sqrt(D)
-    U = C @ (V * (1.0 / S))
-    return (np.ascontiguousarray(U[:, ::-1]), np.ascontiguousarray(S[::-1]), np.ascontiguousarray(V[:, ::-1]))
-
-
 # Product arithmetic of xw / xty: "h2" (FP16 two-term splits on FP16 MFMA)
 # or "f32" (FP32 MFMA).
 PRODUCTS = "h2"
+
+
+def _gram(Cd: torch.Tensor) -> np.ndarray:
+    """Cd^T Cd (L x L, FP64, fixed summation order) for a tall FP64 Cd."""
+    from .als_gpu import _tn
+    L = Cd.shape[1]
+    rows = [_tn(Cd[:, r:r + 64], Cd) for r in range(0, L, 64)]   # the kernel takes a <= 64
+    return torch.cat(rows, dim=0).cpu().numpy()
+
+
+def _times_small(Cd: torch.Tensor, W: np.ndarray) -> torch.Tensor:
+    """Cd W (FP64) for a tall Cd (n x L) and a small host W (L x b)."""
+    from .als_gpu import _nn
+    Wd = torch.from_numpy(np.ascontiguousarray(W)).to(Cd.device)
+    out = None
+    for r in range(0, Cd.shape[1], 64):                           # a <= 64 per product
+        part = _nn(Cd[:, r:r + 64], Wd[r:r + 64])
+        out = part if out is None else out + part
+    return out
+
+
+def eigSVD_dev(C: torch.Tensor):
+    """eigSVD (svd.py:13-28, svd_gpu.py:11-39) with C (n x L, FP32) on the
+    device: the Gram matrix C^T C and U = C V S^-1 are FP64 device products
+    (adx_gemm_tn_f64 / _nn_f64), the L x L eigh runs on the host.  Returns U
+    (device FP32) and host S (FP32), V (FP64), descending order."""
+    Cd = C.double()
+    D, V = np.linalg.eigh(_gram(Cd))
+    V = np.ascontiguousarray(V[:, ::-1])
+    S = np.sqrt(D[::-1])
+    U = _times_small(Cd, V / S).float()
+    return U, S.astype(np.float32), V
+
 
 _WORK: dict = {}
 
@@ -130,15 +156,17 @@ def _SVD_gpu_once(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: 
     rng = np.random.default_rng(seed)
     omega = torch.from_numpy(rng.standard_normal(size=(M, L), dtype=np.float32)).to(dev)
 
-    accum = xty(G, f, omega).cpu().numpy()
-    orth, _, _ = eigSVD_host(accum)
+    accum = xty(G, f, omega)
+    del omega
+    orth, _, _ = eigSVD_dev(accum)
     alpha = 0.0
     sv = np.zeros(L, dtype=np.float32)
     for i in range(power):
-        proj = xw(G, f, torch.from_numpy(orth).to(dev))
-        accum = xty(G, f, proj).cpu().numpy()
-        accum -= alpha * orth
-        orth, s_vals, _ = eigSVD_host(accum)
+        proj = xw(G, f, orth)
+        accum = xty(G, f, proj)
+        del proj
+        accum -= np.float32(alpha) * orth
+        orth, s_vals, _ = eigSVD_dev(accum)
         if i > 0:
             s_cur = s_vals + alpha
             rel = np.abs(s_cur - sv[:len(s_cur)]) / np.maximum(s_cur, 1e-12)
@@ -150,11 +178,11 @@ def _SVD_gpu_once(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: 
             sv[:len(s_vals)] = s_vals + alpha
         if s_vals[-1] > alpha:
             alpha = 0.5 * (s_vals[-1] + alpha)
-    proj = xw(G, f, torch.from_numpy(orth).to(dev)).cpu().numpy()
-    u, s_all, v = eigSVD_host(proj)
-    U = torch.from_numpy(np.ascontiguousarray(u[:, :k])).to(dev)
+    proj = xw(G, f, orth)
+    u, s_all, v = eigSVD_dev(proj)
+    U = u[:, :k].contiguous()
     S = torch.from_numpy(np.ascontiguousarray(s_all[:k])).to(dev)
-    V = torch.from_numpy(np.ascontiguousarray(orth @ v[:, :k])).to(dev)
+    V = _times_small(orth.double(), v[:, :k]).float()
     log.info(f"\n    Total time for SVD={time.time() - t0:.3f}s")
     return U, S, V
 
