@@ -4,11 +4,10 @@
 // N rows), Z (V^T I_q) (M x L times L x K).  General GEMM libraries tile for
 // large square blocks; here
 //   gemm_tn:  C = alpha X^T Y (a x b), X (n x a), Y (n x b): each workgroup
-//             reduces a range of rows with the a x b partial sums spread over
-//             its threads (row chunks staged in LDS) and stores them in its
-//             slice of the caller's workspace; one more kernel sums the
-//             slices in a fixed order (deterministic; without a workspace,
-//             one FP64 atomic per output and workgroup);
+//             reduces a range of rows on FP64 MFMA tiles and stores its
+//             a x b partial sums in its slice of the caller's workspace; one
+//             more kernel sums the slices in a fixed order (deterministic;
+//             without a workspace, one FP64 atomic per output and workgroup);
 //   gemm_nn:  C = alpha X W + u v^T (+ beta C), X (n x a), W (a x b) small:
 //             W (and v) staged in LDS, one thread per (row, output column
 //             group), the row of X read once per thread group.
@@ -20,7 +19,7 @@
 
 namespace adx {
 
-constexpr int TN_ROWS = 32;   // rows per LDS chunk
+constexpr int TN_ROWS = 32;   // row granule of a gemm_tn workgroup's range
 
 // C[i][j] = 0 for i < a, j < b (a kernel, not a memset: graph-safe).
 __global__ void zero_block_kernel(double* __restrict__ C, int a, int b, int64_t ldc) {
@@ -28,55 +27,100 @@ __global__ void zero_block_kernel(double* __restrict__ C, int a, int b, int64_t 
     if (e < a * b) C[(int64_t)(e / b) * ldc + e % b] = 0.0;
 }
 
-// C[i][j] += alpha sum_r X[r][i] Y[r][j]; thread t owns outputs t, t + 256, ...
+// C[i][j] (+)= alpha sum_r X[r][i] Y[r][j] on 16x16x4 FP64 MFMA tiles: the
+// a x b output is TA x TB tiles of 16 x 16; lane (g, c) of a wave supplies
+// X[r + g][16 ta + c] and Y[r + g][16 tb + c] for the 4-row k-step at r
+// (A = X^T, B = Y, as the MFMA operand layouts), and holds the accumulator
+// rows 16 ta + g + 4 j, column 16 tb + c.  The four waves of a workgroup
+// take interleaved 4-row groups of its row range (together 16 consecutive
+// rows per step) and are summed in a fixed order, (w0 + w2) + (w1 + w3); the
+// workgroup's tiles go to its slice of part (or, without a workspace, into C
+// with one FP64 atomic per element).
+template <int TA, int TB>
 __global__ __launch_bounds__(256) void gemm_tn_kernel(const double* __restrict__ X, int64_t ldx,
                                                        const double* __restrict__ Y, int64_t ldy, int64_t n, int a,
                                                        int b, double alpha, int64_t rows_per_block,
                                                        double* __restrict__ C, int64_t ldc,
                                                        double* __restrict__ part) {
-    __shared__ double sx[TN_ROWS][64];
-    __shared__ double sy[TN_ROWS][64];
-    const int tid = threadIdx.x;
-    const int nout = a * b;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};          // a * b <= 1024 = 4 x 256
+    using MM = Mma16<double>;
+    using acc_t = MM::acc;
+    constexpr int NT = TA * TB;
+    __shared__ double red[2][NT][256];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r1 = r0 + rows_per_block < n ? r0 + rows_per_block : n;
-    for (int64_t c0 = r0; c0 < r1; c0 += TN_ROWS) {
-        const int nr = (int)(r1 - c0 < TN_ROWS ? r1 - c0 : TN_ROWS);
-        __syncthreads();
-        for (int e = tid; e < TN_ROWS * 64; e += 256) {
-            const int rr = e >> 6, c = e & 63;
-            sx[rr][c] = (rr < nr && c < a) ? X[(c0 + rr) * ldx + c] : 0.0;
-            sy[rr][c] = (rr < nr && c < b) ? Y[(c0 + rr) * ldy + c] : 0.0;
-        }
-        __syncthreads();
+    acc_t acc[NT];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int o = tid + 256 * q;
-            if (o < nout) {
-                const int i = o / b, j = o - i * b;
-                double s = 0.0;
-                for (int rr = 0; rr < nr; ++rr) s += sx[rr][i] * sy[rr][j];
-                acc[q] += s;
-            }
-        }
-    }
+    for (int t = 0; t < NT; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    bool xok[TA], yok[TB];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int o = tid + 256 * q;
-        if (o < nout) {
-            if (part) {
-                part[(int64_t)blockIdx.x * nout + o] = alpha * acc[q];
-            } else {
-                const int i = o / b, j = o - i * b;
-                atomicAdd(&C[i * ldc + j], alpha * acc[q]);
-            }
+    for (int ta = 0; ta < TA; ++ta) xok[ta] = 16 * ta + cl < a;
+#pragma unroll
+    for (int tb = 0; tb < TB; ++tb) yok[tb] = 16 * tb + cl < b;
+    // two 4-row groups per iteration (rows r + g and r + 16 + g of this wave)
+    for (int64_t r = r0 + 4 * wave; r < r1; r += 32) {
+        double xv[2][TA], yv[2][TB];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int64_t row = r + 16 * u + g;
+            const bool live = row < r1;
+#pragma unroll
+            for (int ta = 0; ta < TA; ++ta) xv[u][ta] = (live && xok[ta]) ? X[row * ldx + 16 * ta + cl] : 0.0;
+#pragma unroll
+            for (int tb = 0; tb < TB; ++tb) yv[u][tb] = (live && yok[tb]) ? Y[row * ldy + 16 * tb + cl] : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+                for (int tb = 0; tb < TB; ++tb)
+                    acc[ta * TB + tb] = MM::run(xv[u][ta], yv[u][tb], acc[ta * TB + tb]);
     }
+    if (wave >= 2) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[wave - 2][t][64 * j + lane] = acc[t][j];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[t][j] += red[wave][t][64 * j + lane];
+    }
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[0][t][64 * j + lane] = acc[t][j];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    const int nout = a * b;
+#pragma unroll
+    for (int ta = 0; ta < TA; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int t = ta * TB + tb;
+                const int i = 16 * ta + MM::row(g, j), jc = 16 * tb + cl;
+                if (i < a && jc < b) {
+                    const double v = alpha * (acc[t][j] + red[0][t][64 * j + lane]);
+                    if (part)
+                        part[(int64_t)blockIdx.x * nout + i * b + jc] = v;
+                    else
+                        atomicAdd(&C[i * ldc + jc], v);
+                }
+            }
 }
 
-// C[i][j] = sum over the nblk slices of part[blk * a b + i b + j]: one
-// wavefront per output, lane l adds slices l, l + 64, ... in order, then a
 // fixed shuffle tree (deterministic).
 __global__ __launch_bounds__(64) void gemm_tn_reduce_kernel(const double* __restrict__ part, int64_t nblk, int a,
                                                             int b, double* __restrict__ C, int64_t ldc) {
@@ -139,17 +183,25 @@ int adx_gemm_tn_f64(const double* X, int64_t ldx, const double* Y, int64_t ldy, 
     if (n > 0) {
         int64_t blocks = ceil_div(n, 4 * TN_ROWS);
         if (blocks > ADX_GEMM_TN_BLOCKS) blocks = ADX_GEMM_TN_BLOCKS;
-        rpb = ceil_div(ceil_div(n, blocks), TN_ROWS) * TN_ROWS;
+        rpb = ceil_div(ceil_div(n, blocks), TN_ROWS) * TN_ROWS;   // a multiple of the 16 rows of one step
         nblk = ceil_div(n, rpb);
     }
+    const int TA = (a + 15) / 16, TB = (b + 15) / 16;   // a b <= 1024: TA TB <= 8
+    auto launch = [&](double* part) {
+#define ADX_TN(ta, tb)                                                                                        \
+    if (TA == ta && TB == tb)                                                                                 \
+        gemm_tn_kernel<ta, tb><<<(unsigned)nblk, 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc, part);
+        ADX_TN(1, 1) ADX_TN(1, 2) ADX_TN(1, 3) ADX_TN(1, 4) ADX_TN(2, 1) ADX_TN(2, 2) ADX_TN(2, 3) ADX_TN(2, 4)
+        ADX_TN(3, 1) ADX_TN(3, 2) ADX_TN(4, 1) ADX_TN(4, 2)
+#undef ADX_TN
+    };
     if (work && nblk > 0) {
-        gemm_tn_kernel<<<(unsigned)nblk, 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc, work);
+        launch(work);
         gemm_tn_reduce_kernel<<<(unsigned)(a * b), 64, 0, st>>>(work, nblk, a, b, C, ldc);
     } else {
         // C is overwritten: zero it with a kernel (graph-safe), then accumulate.
         zero_block_kernel<<<(unsigned)ceil_div(a * b, 256), 256, 0, st>>>(C, a, b, ldc);
-        if (nblk > 0)
-            gemm_tn_kernel<<<(unsigned)nblk, 256, 0, st>>>(X, ldx, Y, ldy, n, a, b, alpha, rpb, C, ldc, nullptr);
+        if (nblk > 0) launch(nullptr);
     }
     ADX_LAUNCH_CHECK("adx_gemm_tn_f64");
     return 0;

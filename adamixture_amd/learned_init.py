@@ -87,6 +87,24 @@ def init_params(seed: int = 0) -> np.ndarray:
 # features and the P that goes with a Q warm start
 # ---------------------------------------------------------------------------
 
+def _tall_tn(X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+    """X^T Y for tall FP64 X, Y: on the GPU the skinny-product kernel
+    (adx_gemm_tn_f64, split reduction in a fixed order; a library GEMM runs
+    a 500,000-long reduction on a handful of workgroups)."""
+    if X.is_cuda:
+        from .als_gpu import _tn
+        return _tn(X.contiguous(), Y.contiguous())
+    return X.T @ Y
+
+
+def _tall_nn(X: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """X W for tall FP64 X and a small W (adx_gemm_nn_f64 on the GPU)."""
+    if X.is_cuda:
+        from .als_gpu import _nn
+        return _nn(X.contiguous(), W.contiguous())
+    return X @ W
+
+
 def features(U2: torch.Tensor, S2: torch.Tensor, V2: torch.Tensor, f: torch.Tensor, P_als: torch.Tensor,
              Q_als: torch.Tensor, K: int) -> torch.Tensor:
     """Per-sample features (N x 64, FP32) from the rank-2K SVD (U2 M x 2K,
@@ -102,9 +120,11 @@ def features(U2: torch.Tensor, S2: torch.Tensor, V2: torch.Tensor, f: torch.Tens
     X = torch.zeros((N, D_IN), dtype=torch.float32, device=dev)
     Qa = Q_als.to(dev, dt)
     X[:, :K] = Qa.to(torch.float32)
-    I_p = P @ _pinv_host(P.T @ P)
-    Z2 = U2 * S2
-    X[:, 16:16 + K] = (0.5 * (V2 @ (Z2.T @ I_p)) + f.reshape(1, -1) @ I_p).to(torch.float32)
+    I_p = _tall_nn(P, _pinv_host(_tall_tn(P, P)))
+    Z2 = (U2 * S2).contiguous()
+    ZtI = _tall_tn(Z2, I_p)
+    fI = _tall_tn(f.reshape(-1, 1).contiguous(), I_p)[0]
+    X[:, 16:16 + K] = (0.5 * _tall_nn(V2.contiguous(), ZtI) + fI).to(torch.float32)
     X[:, 32:32 + K] = (torch.log(Qa.clamp_min(1e-5)) / math.log(1e-5)).to(torch.float32)
     s = V2 * S2
     e = (s * s).sum(dim=1)
@@ -144,12 +164,12 @@ def p_from_q_data(G: PackedGenotypes, f: torch.Tensor, Q: torch.Tensor) -> torch
     from .solvers import batch_bvls_bpp
     from .svd_gpu import xw
     dt = torch.float64
-    Q = Q.to(dt)
-    A_q = Q.T @ Q
-    I_q = Q @ _pinv_host(A_q)
+    Q = Q.to(dt).contiguous()
+    A_q = _tall_tn(Q, Q)
+    I_q = _tall_nn(Q, _pinv_host(A_q))
     Y = xw(G, f.to(torch.float32).contiguous(), I_q.to(torch.float32).contiguous()).to(dt)
     P_free = 0.5 * Y + torch.outer(f.to(dt), I_q.sum(dim=0))
-    P = batch_bvls_bpp(A_q, P_free @ A_q, 0.0, 1.0)
+    P = batch_bvls_bpp(A_q, _tall_nn(P_free, A_q), 0.0, 1.0)
     mapP(P)
     return P
 
