@@ -36,6 +36,20 @@ def mapQ(Q: torch.Tensor) -> None:
     Q.div_(Q.sum(dim=1, keepdim=True))
 
 
+def _host_inverses(a: np.ndarray, dev, with_inv: bool):
+    """pinv (torch.linalg.pinv's cut-off) and, if asked, inv of a host K x K
+    matrix, both moved to the device: one host copy of the Gram matrix serves
+    the pseudo-inverse of the ALS update and the BVLS start X0 = B inv(A)^T."""
+    pinv = torch.from_numpy(np.linalg.pinv(a, rcond=max(a.shape) * np.finfo(np.float64).eps)).to(dev)
+    if not with_inv:
+        return pinv, None
+    try:
+        inv = np.linalg.inv(a)
+    except np.linalg.LinAlgError as exc:
+        raise RuntimeError(f"batch_bvls_bpp: A is singular ({exc})") from exc
+    return pinv, torch.from_numpy(inv).to(dev).contiguous()
+
+
 def _pinv_host(A: torch.Tensor) -> torch.Tensor:
     """torch.linalg.pinv's cut-off (rtol = max(m, n) eps) on the host: the
     K x K matrix is tiny, and an LAPACK SVD on the CPU is ~10x faster than a
@@ -126,18 +140,33 @@ def ALS_gpu(U: torch.Tensor, S: torch.Tensor, V: torch.Tensor, f: torch.Tensor, 
         Q = _nn(V, ZtI, 0.5, ones_n, fI)
         mapQ(Q)
     Q0 = Q.clone()
+    inv_start = bvls_start == "inverse"
+    # Each half-step needs its K x K Gram matrix on the host (pinv, and inv
+    # for the BVLS start).  The convergence value of an iteration travels
+    # with the next iteration's A_q = Q^T Q, computed before the test (it
+    # depends only on Q), so an iteration costs two host round trips.
+    A_q = _tn(Q, Q)
+    aq = A_q.cpu().numpy()
     for i in range(max_iter):
-        A_q = _tn(Q, Q)
-        I_q = _nn(Q, _pinv_host(A_q))
+        pinv_q, inv_q = _host_inverses(aq, dev, inv_start)
+        I_q = _nn(Q, pinv_q)
         VtI = _tn(V, I_q)
         s = _tn(ones_col, I_q)[0]
-        P = batch_bvls_bpp(A_q, _nn(Z, 0.5 * (VtI @ A_q), 1.0, f_w, s @ A_q), 0.0, 1.0, start=bvls_start)
+        P = batch_bvls_bpp(A_q, _nn(Z, 0.5 * (VtI @ A_q), 1.0, f_w, s @ A_q), 0.0, 1.0, start=bvls_start,
+                           Ainv=inv_q)
         mapP(P)
-        A_p, ZtI, fI = q_side(P)
-        Q = batch_bvls_bpp(A_p, _nn(V, 0.5 * (ZtI @ A_p), 1.0, ones_n, fI @ A_p), 0.0, 1.0, start=bvls_start)
+        A_p = _tn(P, P)
+        pinv_p, inv_p = _host_inverses(A_p.cpu().numpy(), dev, inv_start)
+        I_p = _nn(P, pinv_p)
+        ZtI, fI = _tn(Z, I_p), _tn(f_col, I_p)[0]
+        Q = batch_bvls_bpp(A_p, _nn(V, 0.5 * (ZtI @ A_p), 1.0, ones_n, fI @ A_p), 0.0, 1.0, start=bvls_start,
+                           Ainv=inv_p)
         mapQ(Q)
         rmse = torch.sqrt(torch.mean((Q - Q0) ** 2))
-        if rmse < tol:
+        A_q = _tn(Q, Q)
+        both = torch.cat([rmse.reshape(1), A_q.reshape(-1)]).cpu().numpy()
+        aq = both[1:].reshape(K, K)
+        if both[0] < tol:
             log.info(f"        Convergence reached in iteration {i + 1}.")
             break
         Q0.copy_(Q)

@@ -9,8 +9,7 @@
 //             more kernel sums the slices in a fixed order (deterministic;
 //             without a workspace, one FP64 atomic per output and workgroup);
 //   gemm_nn:  C = alpha X W + u v^T (+ beta C), X (n x a), W (a x b) small:
-//             W (and v) staged in LDS, one thread per (row, output column
-//             group), the row of X read once per thread group.
+//             16-row tiles on FP64 MFMA with W held in registers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -133,38 +132,58 @@ __global__ __launch_bounds__(64) void gemm_tn_reduce_kernel(const double* __rest
     if (threadIdx.x == 0) C[(int64_t)(o / b) * ldc + o % b] = s;
 }
 
-// C[r][j] = alpha sum_i X[r][i] W[i][j] + u[r] v[j] + beta C[r][j]
+// C[r][j] = alpha sum_i X[r][i] W[i][j] + u[r] v[j] + beta C[r][j] on
+// 16x16x4 FP64 MFMA tiles of 16 rows: lane (g, c) supplies X[r0 + c][4 ks +
+// g] (A) and holds W[4 ks + g][c] (B) for the KS k-steps (W stays in
+// registers for the whole grid-stride loop); the accumulator row g + 4 j is
+// row r0 + g + 4 j of C, column c, so the stores of one instruction cover
+// four row segments of C.
+template <int KS>
 __global__ __launch_bounds__(256) void gemm_nn_kernel(const double* __restrict__ X, int64_t ldx, int64_t n, int a,
                                                        int b, const double* __restrict__ W, int64_t ldw,
                                                        double alpha, const double* __restrict__ u,
                                                        const double* __restrict__ v, double beta,
                                                        double* __restrict__ C, int64_t ldc) {
-    __shared__ double sw[64][17];
-    __shared__ double sv[16];
-    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
-        const int i = e >> 4, j = e & 15;
-        sw[i][j] = (i < a && j < b) ? W[i * ldw + j] : 0.0;
+    using MM = Mma16<double>;
+    using acc_t = MM::acc;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4;
+    const int cl = lane & 15;
+    const bool col_ok = cl < b;
+    double wv[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int i = 4 * ks + g;
+        wv[ks] = (i < a && col_ok) ? W[i * ldw + cl] : 0.0;
     }
-    if (threadIdx.x < 16) sv[threadIdx.x] = (v && (int)threadIdx.x < b) ? v[threadIdx.x] : 0.0;
-    __syncthreads();
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    double acc[16];
+    const double vc = (v && col_ok) ? v[cl] : 0.0;
+    const int64_t ntiles = (n + 15) >> 4;
+    const int64_t wave_id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int64_t tile = wave_id; tile < ntiles; tile += nwaves) {
+        const int64_t r0 = tile << 4;
+        const int64_t rx = r0 + cl;
+        double xv[KS];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-    for (int i = 0; i < a; ++i) {
-        const double x = X[r * ldx + i];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) acc[j] += x * sw[i][j];
-    }
-    const double ur = u ? u[r] : 0.0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j)
-        if (j < b) {
-            double c = alpha * acc[j] + ur * sv[j];
-            if (beta != 0.0) c += beta * C[r * ldc + j];
-            C[r * ldc + j] = c;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int i = 4 * ks + g;
+            xv[ks] = (rx < n && i < a) ? X[rx * ldx + i] : 0.0;
         }
+        acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) acc = MM::run(xv[ks], wv[ks], acc);
+        if (!col_ok) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t r = r0 + MM::row(g, j);
+            if (r < n) {
+                double c = alpha * acc[j];
+                if (u) c += u[r] * vc;
+                if (beta != 0.0) c += beta * C[r * ldc + cl];
+                C[r * ldc + cl] = c;
+            }
+        }
+    }
 }
 
 }  // namespace adx
@@ -214,8 +233,20 @@ int adx_gemm_nn_f64(const double* X, int64_t ldx, int64_t n, int a, int b, const
     if (ldx < a || ldw < b || ldc < b) return fail_msg(1, "adx_gemm_nn_f64: bad stride");
     if ((u == nullptr) != (v == nullptr)) return fail_msg(1, "adx_gemm_nn_f64: u and v go together");
     if (n <= 0) return 0;
-    gemm_nn_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, (hipStream_t)stream>>>(X, ldx, n, a, b, W, ldw, alpha, u,
-                                                                               v, beta, C, ldc);
+    // 16-row tiles, 4 per workgroup, grid-stride beyond 4096 workgroups
+    int64_t blocks = ceil_div(ceil_div(n, 16), 4);
+    if (blocks > 4096) blocks = 4096;
+    hipStream_t st = (hipStream_t)stream;
+    const int ks = (a + 3) / 4;
+#define ADX_NN(k) gemm_nn_kernel<k><<<(unsigned)blocks, 256, 0, st>>>(X, ldx, n, a, b, W, ldw, alpha, u, v, beta, C, ldc)
+    if (ks <= 1) ADX_NN(1);
+    else if (ks == 2) ADX_NN(2);
+    else if (ks == 3) ADX_NN(3);
+    else if (ks == 4) ADX_NN(4);
+    else if (ks <= 8) ADX_NN(8);      // k-steps past a multiply zeros
+    else if (ks <= 12) ADX_NN(12);
+    else ADX_NN(16);
+#undef ADX_NN
     ADX_LAUNCH_CHECK("adx_gemm_nn_f64");
     return 0;
 }

@@ -55,10 +55,11 @@ def sqp_solve_p(XtX_p: torch.Tensor, Xtz_p: torch.Tensor, P: torch.Tensor, M: in
 
 
 def batch_bvls_bpp(A: torch.Tensor, B: torch.Tensor, lower: float, upper: float, *,
-                   start: str = "inverse") -> torch.Tensor:
+                   start: str = "inverse", Ainv: torch.Tensor | None = None) -> torch.Tensor:
     """bvls_kernel.cu:188-218 batch_bvls_bpp_cuda.  ``start="inverse"`` begins
     from X0 = B inv(A)^T like the CUDA op; ``start="solve"`` begins from the
-    Gauss-Jordan solution like the Cython batch_bvls_bpp."""
+    Gauss-Jordan solution like the Cython batch_bvls_bpp.  ``Ainv``: inv(A)
+    on the device when the caller already has it (saves a host round trip)."""
     dev = B.device
     M, K = B.shape
     if K > 64:
@@ -67,7 +68,12 @@ def batch_bvls_bpp(A: torch.Tensor, B: torch.Tensor, lower: float, upper: float,
         raise RuntimeError("A must be K x K")
     A = _f64c(A, dev)
     B = _f64c(B, dev)
-    Ainv = _inv_host(A) if start == "inverse" else None
+    if start != "inverse":
+        Ainv = None
+    elif Ainv is None:
+        Ainv = _inv_host(A)
+    else:
+        Ainv = _f64c(Ainv, dev)
     X = torch.empty((M, K), dtype=torch.float64, device=dev)
     if Ainv is not None:
         nb = _lib.load().adx_bvls_workspace_bytes(K)
