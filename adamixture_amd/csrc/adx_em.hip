@@ -55,11 +55,11 @@ __device__ __forceinline__ void em_terms<float>(float h, uint32_t code, float& a
 }
 
 // FP64: the reference's quotients g / H and (2-g) / (1-H)
-// (em_adam_gpu.py:62-67) as g and 2-g times 1/H and 1/(1-H), each
-// reciprocal the hardware estimate refined by two Newton-Raphson steps
-// r <- r + r (1 - x r) (x in [1e-5, 1]); a and b stay within a few ulp of
-// the correctly rounded quotients, the size of a change of summation order
-// in the contractions that follow.
+// (em_adam_gpu.py:62-67) as g and 2-g times 1/H = (1-H) r and 1/(1-H) = H r
+// with r = 1 / (H (1-H)), the hardware estimate refined by two
+// Newton-Raphson steps r <- r + r (1 - x r) (x in [1e-5, 1/4]); a and b stay
+// within a few ulp of the correctly rounded quotients, the size of a change
+// of summation order in the contractions that follow.
 __device__ __forceinline__ double recip_nr(double x) {
     double r = __builtin_amdgcn_rcp(x);
     double e = fma(-x, r, 1.0);
@@ -71,9 +71,11 @@ __device__ __forceinline__ double recip_nr(double x) {
 template <>
 __device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double& a, double& b) {
     const double hc = fmin(fmax(h, 1e-5), 1.0 - 1e-5);
+    const double u = 1.0 - hc;
+    const double r = recip_nr(hc * u);
     const double g = (double)code;
-    a = code == 3u ? 0.0 : g * recip_nr(hc);
-    b = code == 3u ? 0.0 : (2.0 - g) * recip_nr(1.0 - hc);
+    a = code == 3u ? 0.0 : g * (u * r);
+    b = code == 3u ? 0.0 : (2.0 - g) * (hc * r);
 }
 
 template <typename T, int KS, int CW, bool STATS, bool LL>
