@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--tiles", default=None, help="fixed EM tile shape 'cw/target_wgs' (implies --no-autotune)")
     ap.add_argument("--no-ttt", action="store_true",
                     help="skip the time-to-target race (the headline; ~40 s per seed)")
-    ap.add_argument("--ttt-seeds", default="42,43,44", help="initialisation seeds of the race (median reported)")
+    ap.add_argument("--ttt-seeds", default="42,43,44,45,46", help="initialisation seeds of the race (median reported)")
     return ap.parse_args()
 
 
