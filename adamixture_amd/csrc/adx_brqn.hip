@@ -9,10 +9,18 @@
 //   P side (per SNP j):     Xtz_p = sum_i (z1 - z2) q_i
 //                           XtX_p = sum_i (w1 + w2) q_i q_i^T
 // The reference forms these with einsum over C x N x K x K temporaries.  Here
-// each thread owns one problem's K(K+1)/2 + K accumulators in registers and
-// streams the 2-bit genotypes once:
-//   Q side: thread = sample, workgroup = 256 consecutive samples x a SNP
-//           range; one coalesced byte per 4 SNPs, P rows are wave-uniform.
+// the Q side uses (1-p)(1-p)^T = 1 1^T - 1 p^T - p 1^T + p p^T:
+//   XtX_q[k][l] = sum_j (w1 + w2) p_k p_l + S_w2 - T_k - T_l,
+//   Xtz_q[k]    = sum_j (z1 - z2) p_k + S_z2,
+// with S_w2 = sum_j w2, T_k = sum_j w2 p_k, S_z2 = sum_j z2: one FMA per
+// Hessian term and genotype instead of two (the corrections are formed once
+// per problem and reduction split; the rounding of the subtractions is
+// relative to S_w2, the scale of the Hessian entries).  Each thread owns one
+// problem's accumulators in registers and streams the 2-bit genotypes once:
+//   Q side: thread = sample (K <= 8) or two waves per 64 samples, each with
+//           half the Hessian rows (K = 9..16); workgroup = 256 or 128
+//           consecutive samples x a SNP range; one coalesced byte per 4
+//           SNPs, P rows are wave-uniform.
 //   P side: thread = SNP, workgroup = 256 SNPs; 256-sample chunks of the
 //           genotype rows and of Q are staged in LDS (Q reads broadcast).
 // Sample splits / SNP splits are combined with FP64 atomics.
@@ -54,27 +62,14 @@ __device__ __forceinline__ Weights bq_weights(double h, uint32_t code) {
     return w;
 }
 
-template <int KM>
-__global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
-                                                          int64_t N, int K, const double* __restrict__ Q,
-                                                          const double* __restrict__ P, int64_t rows_per_split,
-                                                          double* __restrict__ XtX, double* __restrict__ Xtz) {
-    constexpr int KT = KM * (KM + 1) / 2;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t Mb = (M + 3) >> 2;
-    const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
-    const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
-    const int64_t m_end = 4 * r1 < M ? 4 * r1 : M;     // SNPs of this split: 4 r0 .. m_end - 1
-    const bool live = i < N;
-    double q[KM], hz[KM], hx[KT];
-#pragma unroll
-    for (int k = 0; k < KM; ++k) {
-        q[k] = (live && k < K) ? Q[i * K + k] : 0.0;
-        hz[k] = 0.0;
-    }
-#pragma unroll
-    for (int t = 0; t < KT; ++t) hx[t] = 0.0;
-    if (4 * r0 >= m_end) return;
+// Q side, Hessian rows K0 <= k < K1 of the upper triangle (pairs k <= l <
+// KM, row by row in hx) for sample i over the byte rows r0 .. r1 - 1; with
+// OWN also the gradient sum hz, T (tw) and the sums S_z2, S_w2.
+template <int KM, int K0, int K1, bool OWN>
+__device__ __forceinline__ void gh_q_loop(const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int K,
+                                          const double* __restrict__ P, const double (&q)[KM], int64_t i, bool live,
+                                          int64_t r0, int64_t r1, int64_t m_end, double* hx, double* hz,
+                                          double* tw, double& sz2, double& sw2) {
     // The P row of the next SNP (the same address in every lane) and the
     // genotype byte of the next row are loaded one step ahead.
     double pc[KM];
@@ -105,34 +100,114 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
 #pragma unroll
             for (int k = 0; k < KM; ++k) h += q[k] * pc[k];
             const Weights w = bq_weights(h, code);
-            double a[KM], b[KM];
+            const double ws = w.w1 + w.w2;
+            if (OWN) {
+                const double zd = w.z1 - w.z2;
+                sz2 += w.z2;
+                sw2 += w.w2;
 #pragma unroll
-            for (int k = 0; k < KM; ++k) {
-                const double u = 1.0 - pc[k];
-                hz[k] += w.z1 * pc[k] + w.z2 * u;
-                a[k] = w.w1 * pc[k];
-                b[k] = w.w2 * u;
+                for (int k = 0; k < KM; ++k) {
+                    hz[k] += zd * pc[k];
+                    tw[k] += w.w2 * pc[k];
+                }
             }
             int t = 0;
 #pragma unroll
-            for (int k = 0; k < KM; ++k)
+            for (int k = K0; k < K1; ++k) {
+                const double a = ws * pc[k];
 #pragma unroll
-                for (int l = k; l < KM; ++l, ++t) hx[t] += a[k] * pc[l] + b[k] * (1.0 - pc[l]);
+                for (int l = k; l < KM; ++l, ++t) hx[t] += a * pc[l];
+            }
 #pragma unroll
             for (int k = 0; k < KM; ++k) pc[k] = k < K ? pn[k] : 0.0;
         }
     }
-    if (!live) return;
+}
+
+// Adds rows K0 .. K1 - 1 of sample i's Hessian (with the rank-one
+// corrections) and, with OWN, its gradient to the outputs.
+template <int KM, int K0, int K1, bool OWN>
+__device__ __forceinline__ void gh_q_store(int K, int64_t i, const double* hx, const double* hz, const double* tw,
+                                           double sz2, double sw2, double* __restrict__ XtX,
+                                           double* __restrict__ Xtz) {
+    if (OWN) {
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
+            if (k < K) atomicAdd(&Xtz[i * K + k], hz[k] + sz2);
+    }
     int t = 0;
 #pragma unroll
-    for (int k = 0; k < KM; ++k) {
-        if (k < K) atomicAdd(&Xtz[i * K + k], hz[k]);
+    for (int k = K0; k < K1; ++k)
 #pragma unroll
         for (int l = k; l < KM; ++l, ++t)
             if (k < K && l < K) {
-                atomicAdd(&XtX[(i * K + k) * K + l], hx[t]);
-                if (l != k) atomicAdd(&XtX[(i * K + l) * K + k], hx[t]);
+                const double v = ((hx[t] + sw2) - tw[k]) - tw[l];
+                atomicAdd(&XtX[(i * K + k) * K + l], v);
+                if (l != k) atomicAdd(&XtX[(i * K + l) * K + k], v);
             }
+}
+
+// Hessian pairs of rows K0 .. K1 - 1 of a KM x KM upper triangle.
+constexpr int tri_pairs(int KM, int K0, int K1) { return K0 >= K1 ? 0 : (KM - K0) + tri_pairs(KM, K0 + 1, K1); }
+
+// KS = 0: one thread per sample, 256 samples per workgroup.  KS > 0: two
+// waves per 64 samples, 128 samples per workgroup; the even wave owns rows
+// 0 .. KS - 1, the gradient, T and the sums, the odd wave rows KS .. KM - 1,
+// and reads T and S_w2 of its samples from LDS after the reduction (the two
+// roles have about equal work per genotype, with half the registers of one
+// thread owning the whole triangle).
+template <int KM, int KS>
+__global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restrict__ G, int64_t ldg, int64_t M,
+                                                          int64_t N, int K, const double* __restrict__ Q,
+                                                          const double* __restrict__ P, int64_t rows_per_split,
+                                                          double* __restrict__ XtX, double* __restrict__ Xtz) {
+    constexpr int PA = tri_pairs(KM, 0, KS > 0 ? KS : KM);
+    constexpr int PB = KS > 0 ? tri_pairs(KM, KS, KM) : 0;
+    constexpr int PX = PA > PB ? PA : PB;
+    __shared__ double s_corr[KS > 0 ? 128 : 1][KM + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int role = KS > 0 ? (wave & 1) : 0;
+    const int slot = KS > 0 ? (wave >> 1) * 64 + lane : 0;
+    const int64_t i = KS > 0 ? (int64_t)blockIdx.x * 128 + slot : (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t Mb = (M + 3) >> 2;
+    const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
+    const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
+    const int64_t m_end = 4 * r1 < M ? 4 * r1 : M;     // SNPs of this split: 4 r0 .. m_end - 1
+    if (4 * r0 >= m_end) return;                        // the whole workgroup
+    const bool live = i < N;
+    double q[KM], hx[PX], hz[KM], tw[KM];
+    double sz2 = 0.0, sw2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        q[k] = (live && k < K) ? Q[i * K + k] : 0.0;
+        hz[k] = 0.0;
+        tw[k] = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < PX; ++t) hx[t] = 0.0;
+    if (KS == 0) {
+        gh_q_loop<KM, 0, KM, true>(G, ldg, M, K, P, q, i, live, r0, r1, m_end, hx, hz, tw, sz2, sw2);
+        if (live) gh_q_store<KM, 0, KM, true>(K, i, hx, hz, tw, sz2, sw2, XtX, Xtz);
+        return;
+    }
+    constexpr int KA = KS > 0 ? KS : KM;
+    if (role == 0)
+        gh_q_loop<KM, 0, KA, true>(G, ldg, M, K, P, q, i, live, r0, r1, m_end, hx, hz, tw, sz2, sw2);
+    else
+        gh_q_loop<KM, KA, KM, false>(G, ldg, M, K, P, q, i, live, r0, r1, m_end, hx, hz, tw, sz2, sw2);
+    if (role == 0) {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) s_corr[slot][k] = tw[k];
+        s_corr[slot][KM] = sw2;
+    }
+    __syncthreads();
+    if (!live) return;
+    if (role == 0) {
+        gh_q_store<KM, 0, KA, true>(K, i, hx, hz, tw, sz2, sw2, XtX, Xtz);
+    } else {
+#pragma unroll
+        for (int k = 0; k < KM; ++k) tw[k] = s_corr[slot][k];
+        gh_q_store<KM, KA, KM, false>(K, i, hx, hz, tw, 0.0, s_corr[slot][KM], XtX, Xtz);
     }
 }
 
@@ -246,6 +321,7 @@ __global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __re
     __shared__ double s_op[GHW_CH * GHW_LD];   // chunk operand rows (zero past K and past the end)
     __shared__ double s_own[4][64];            // each wave's own row (q_i or p_j)
     __shared__ double s_w[4][4][GHW_CH];       // per wave: weights of the chunk rows
+    __shared__ double s_tw[4][64];             // Q side: each wave's T_k = sum_j w2 p_k
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t nprob = SIDE == 0 ? N : M;   // problems: samples (Q side) or SNPs (P side)
@@ -270,13 +346,17 @@ __global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __re
         tl[u] = bk < nb ? 4 * (bk + t) : -1;
     }
     double hx[GHW_TASKS][16];
-    double hz[4];
+    double hz[4], tw[4];
+    double sz2 = 0.0, sw2 = 0.0;   // Q side: S_z2, S_w2 (every lane alike)
 #pragma unroll
     for (int u = 0; u < GHW_TASKS; ++u)
 #pragma unroll
         for (int e = 0; e < 16; ++e) hx[u][e] = 0.0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) hz[a] = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        hz[a] = 0.0;
+        tw[a] = 0.0;
+    }
 
     for (int64_t r0 = 0; r0 < nrow; r0 += GHW_CH) {
         __syncthreads();
@@ -297,9 +377,9 @@ __global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __re
             for (int k = 0; k < K; ++k) h += s_own[wave][k] * s_op[lane * GHW_LD + k];
             const Weights w = bq_weights(h, code);
             if (SIDE == 0) {
-                s_w[wave][0][lane] = w.z1;
+                s_w[wave][0][lane] = w.z1 - w.z2;
                 s_w[wave][1][lane] = w.z2;
-                s_w[wave][2][lane] = w.w1;
+                s_w[wave][2][lane] = w.w1 + w.w2;
                 s_w[wave][3][lane] = w.w2;
             } else {
                 s_w[wave][0][lane] = w.z1 - w.z2;
@@ -311,31 +391,32 @@ __global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __re
         for (int j = 0; j < nr; ++j) {
             const double* x = s_op + j * GHW_LD;
             if (SIDE == 0) {
-                // XtX_q += w1 p p^T + w2 (1-p)(1-p)^T, Xtz_q += z1 p + z2 (1-p)
-                const double z1 = s_w[wave][0][j], z2 = s_w[wave][1][j];
-                const double w1 = s_w[wave][2][j], w2 = s_w[wave][3][j];
+                // XtX_q += (w1 + w2) p p^T, Xtz_q += (z1 - z2) p, and the
+                // sums S_z2, S_w2, T of the (1-p) terms
+                const double zd = s_w[wave][0][j], z2 = s_w[wave][1][j];
+                const double ws = s_w[wave][2][j], w2 = s_w[wave][3][j];
+                sz2 += z2;
+                sw2 += w2;
                 if (lane < 16)
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
                         const double pk = x[(4 * lane + a) & 63];
-                        hz[a] += z1 * pk + z2 * (1.0 - pk);
+                        hz[a] += zd * pk;
+                        tw[a] += w2 * pk;
                     }
 #pragma unroll
                 for (int u = 0; u < GHW_TASKS; ++u) {
                     if (tk[u] < 0) continue;
-                    double ak[4], bk[4], pl[4], ul[4];
+                    double ak[4], pl[4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
-                        const double pk = x[tk[u] + a];
-                        ak[a] = w1 * pk;
-                        bk[a] = w2 * (1.0 - pk);
+                        ak[a] = ws * x[tk[u] + a];
                         pl[a] = x[tl[u] + a];
-                        ul[a] = 1.0 - pl[a];
                     }
 #pragma unroll
                     for (int a = 0; a < 4; ++a)
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) hx[u][4 * a + b] += ak[a] * pl[b] + bk[a] * ul[b];
+                        for (int b = 0; b < 4; ++b) hx[u][4 * a + b] += ak[a] * pl[b];
                 }
             } else {
                 // XtX_p += (w1 + w2) q q^T, Xtz_p += (z1 - z2) q
@@ -360,11 +441,17 @@ __global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __re
             }
         }
     }
+    if (SIDE == 0) {   // T_k of this wave's problem for every lane's blocks
+        if (lane < 16)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) s_tw[wave][4 * lane + a] = tw[a];
+        __syncthreads();
+    }
     if (!live) return;
     if (lane < 16)
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-            if (4 * lane + a < K) Xtz[prob * K + 4 * lane + a] = hz[a];
+            if (4 * lane + a < K) Xtz[prob * K + 4 * lane + a] = SIDE == 0 ? hz[a] + sz2 : hz[a];
     double* H = XtX + prob * (int64_t)K * K;
 #pragma unroll
     for (int u = 0; u < GHW_TASKS; ++u) {
@@ -375,8 +462,10 @@ __global__ __launch_bounds__(256) void grad_hess_wide_kernel(const uint8_t* __re
             for (int b = 0; b < 4; ++b) {
                 const int k = tk[u] + a, l = tl[u] + b;
                 if (k < K && l < K) {
-                    H[k * K + l] = hx[u][4 * a + b];
-                    H[l * K + k] = hx[u][4 * a + b];
+                    const double v = SIDE == 0 ? ((hx[u][4 * a + b] + sw2) - s_tw[wave][k]) - s_tw[wave][l]
+                                               : hx[u][4 * a + b];
+                    H[k * K + l] = v;
+                    H[l * K + k] = v;
                 }
             }
     }
@@ -388,13 +477,16 @@ static void launch_gh(int side, hipStream_t st, const uint8_t* G, int64_t ldg, i
                       const double* Q, const double* P, double* XtX, double* Xtz) {
     const int64_t Mb = ceil_div(M, 4);
     if (side == 0) {
-        const int64_t nbx = ceil_div(N, 256);
+        // rows of the Hessian triangle owned by the first of two waves (0: one
+        // thread per sample); chosen so both roles do about equal work
+        constexpr int KS = KM <= 8 ? 0 : KM <= 10 ? 2 : KM <= 14 ? 3 : 4;
+        const int64_t nbx = ceil_div(N, KS > 0 ? 128 : 256);
         int64_t s = ceil_div(4096, nbx);
         s = s > Mb ? Mb : (s < 1 ? 1 : s);
         if (s > 65535) s = 65535;
         const int64_t rps = ceil_div(Mb, s);
         dim3 grid((unsigned)nbx, (unsigned)ceil_div(Mb, rps));
-        grad_hess_q_kernel<KM><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, rps, XtX, Xtz);
+        grad_hess_q_kernel<KM, KS><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Q, P, rps, XtX, Xtz);
     } else {
         const int64_t nbx = ceil_div(M, 256);
         int64_t s = ceil_div(2048, nbx);
