@@ -374,9 +374,10 @@ def roofline(prec, kern_ms, w):
 def brqn_bench(G, P0, Q0, w, dev):
     """The reference's default optimizer (SQP block relaxation + ZAL
     quasi-Newton): its Q- and P-side gradient / Hessian passes timed with HIP
-    events (algorithmic flops per genotype: H 2K, gradient 4K / 2K, Hessian
-    4 K(K+1)/2 / 2 K(K+1)/2 on the Q / P side, against the FP64 peak), and
-    two whole iterations from the seeded random start."""
+    events (algorithmic flops per genotype: H 2K, gradient 2K, Hessian
+    K(K+1) on both sides, plus 2K for the Q side's rank-one terms sum w2 p,
+    against the FP64 peak), and two whole iterations from the seeded random
+    start."""
     import torch
 
     from adamixture_amd.br_qn_gpu import grad_hess, optimize_original_gpu
@@ -384,7 +385,7 @@ def brqn_bench(G, P0, Q0, w, dev):
     P = torch.from_numpy(P0).to(dev)
     Q = torch.from_numpy(Q0).to(dev)
     out = {}
-    for side, per in (("q", 2 * K + 4 * K + 2 * K * (K + 1)), ("p", 2 * K + 2 * K + K * (K + 1))):
+    for side, per in (("q", 2 * K + 2 * K + 2 * K + K * (K + 1)), ("p", 2 * K + 2 * K + K * (K + 1))):
         grad_hess(G, Q, P, side)
         ms = []
         for _ in range(2):
