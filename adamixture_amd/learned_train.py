@@ -203,15 +203,20 @@ class _Reached(Exception):
 
 
 def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_precision: str = "f32",
-         fast_precision: str = "bf16", rel: float = 1e-6, extra_plain: tuple = ()) -> dict:
+         fast_precision: str = "bf16", rel: float = 1e-6, extra_plain: tuple = (), omega: str = "device") -> dict:
     """Time from the start of initialisation to the first logL check within
     ``rel`` of L*, where L* is the FP64 logL of the converged plain GPU run
     (randomized SVD + ALS start, Adam-EM in ``plain_precision``).  Races:
     plain start in ``plain_precision`` (defines L*), plain start in
     ``fast_precision``, and (with a model) the learned start in
     ``fast_precision``.  Each precision in ``extra_plain`` (e.g. "f64", the
-    reference's arithmetic) races the plain start too, stopped at the target."""
-    from .svd_gpu import SVD_gpu as _svd
+    reference's arithmetic) races the plain start too, stopped at the target.
+    ``omega``: where every start's SVD draws its test matrix (svd_gpu.SVD_gpu;
+    "device" as the reference GPU path, the same for all starts)."""
+    from functools import partial
+
+    from .svd_gpu import SVD_gpu
+    _svd = partial(SVD_gpu, omega=omega)
     from .learned_init import learned_warm_start
     dev = torch.device(device)
     out = {"K": K, "rel": rel}

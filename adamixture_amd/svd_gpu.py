@@ -14,8 +14,10 @@ eigen-decomposition (L = max(k + 10, 20)) runs on the host in LAPACK.
 The Gaussian test matrix is drawn on the host with
 ``numpy.random.default_rng(seed).standard_normal((M, L), float32)`` exactly
 as svd.py:52-62 does, so the GPU result reproduces the CPU reference's
-numbers (the reference GPU path draws from torch's device generator,
-svd_gpu.py:68-71, which no other device reproduces).
+numbers (``omega="host"``, the default).  ``omega="device"`` draws it from a
+seeded torch device generator as the reference GPU path does
+(svd_gpu.py:68-71): deterministic on one GPU model, without the host draw
+(~0.05 s at M=500k, L=20), but not reproducible on other devices.
 """
 from __future__ import annotations
 
@@ -145,16 +147,26 @@ def xty(G: PackedGenotypes, f: torch.Tensor, Y: torch.Tensor, products: str | No
     return A
 
 
+def _test_matrix(M: int, L: int, seed: int, dev, omega: str) -> torch.Tensor:
+    if omega == "host":
+        rng = np.random.default_rng(seed)
+        return torch.from_numpy(rng.standard_normal(size=(M, L), dtype=np.float32)).to(dev)
+    if omega == "device":
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        return torch.randn((M, L), dtype=torch.float32, generator=gen, device=dev)
+    raise ValueError(f"omega must be 'host' or 'device' (got {omega!r})")
+
+
 def _SVD_gpu_once(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: int, tol: float,
-                  chunk_size: int, device, threads_per_block: int = 0):
+                  chunk_size: int, device, threads_per_block: int = 0, omega: str = "host"):
     """svd.py:30-140 / svd_gpu.py:40-154 (dashSVD with dynamic shift)."""
     t0 = time.time()
     dev = torch.device(device)
     G = as_packed(G, M, dev)
     f = f.to(device=dev, dtype=torch.float32).contiguous()
     L = max(k + 10, 20)
-    rng = np.random.default_rng(seed)
-    omega = torch.from_numpy(rng.standard_normal(size=(M, L), dtype=np.float32)).to(dev)
+    omega = _test_matrix(M, L, seed, dev, omega)
 
     accum = xty(G, f, omega)
     del omega
@@ -188,14 +200,14 @@ def _SVD_gpu_once(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: 
 
 
 def SVD_gpu(G, N: int, M: int, f: torch.Tensor, k: int, seed: int, power: int, tol: float, chunk_size: int,
-            device, threads_per_block: int = 0):
+            device, threads_per_block: int = 0, *, omega: str = "host"):
     """svd_gpu.py:157-178 with the OOM retry (no chunk temporaries here, so a
     retry can only come from the O(M L) buffers)."""
     from .utils import MIN_GPU_CHUNK_SIZE, is_gpu_oom, reduce_gpu_chunk_or_raise
     cur = chunk_size
     while True:
         try:
-            return _SVD_gpu_once(G, N, M, f, k, seed, power, tol, cur, device, threads_per_block)
+            return _SVD_gpu_once(G, N, M, f, k, seed, power, tol, cur, device, threads_per_block, omega)
         except RuntimeError as exc:
             if not is_gpu_oom(exc) or cur <= MIN_GPU_CHUNK_SIZE:
                 raise

@@ -130,6 +130,22 @@ def test_svd_matches_cpu_reference_fixture(demo, fixtures):
     np.testing.assert_allclose(V, fixtures["cpu_brqn_svd_V"], atol=1e-4)
 
 
+def test_svd_device_test_matrix(demo):
+    """omega="device" (a seeded torch device generator, as the reference GPU
+    path draws it): reruns are identical, and the leading singular values
+    agree with the host-drawn SVD's (a different random start of the same
+    power iteration)."""
+    G, N, M = demo
+    f = calculate_frequencies_gpu(G, M, 8192, G.device)
+    U1, S1, V1 = SVD_gpu(G, N, M, f, 7, 42, 5, 0.1, 8192, G.device, omega="device")
+    U2, S2, V2 = SVD_gpu(G, N, M, f, 7, 42, 5, 0.1, 8192, G.device, omega="device")
+    assert torch.equal(U1, U2) and torch.equal(S1, S2) and torch.equal(V1, V2)
+    Uh, Sh, Vh = SVD_gpu(G, N, M, f, 7, 42, 5, 0.1, 8192, G.device)
+    np.testing.assert_allclose(S1[:3].cpu().numpy(), Sh[:3].cpu().numpy(), rtol=1e-3)
+    with pytest.raises(ValueError, match="omega"):
+        SVD_gpu(G, N, M, f, 7, 42, 5, 0.1, 8192, G.device, omega="gpu")
+
+
 @pytest.mark.parametrize("start", ["solve", "inverse"])
 def test_als_matches_cpu_reference_fixture(demo, fixtures, start):
     G, N, M = demo
