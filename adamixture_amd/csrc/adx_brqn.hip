@@ -64,62 +64,62 @@ __device__ __forceinline__ Weights bq_weights(double h, uint32_t code) {
 
 // Q side, Hessian rows K0 <= k < K1 of the upper triangle (pairs k <= l <
 // KM, row by row in hx) for sample i over the byte rows r0 .. r1 - 1; with
-// OWN also the gradient sum hz, T (tw) and the sums S_z2, S_w2.
+// OWN also the gradient sum hz, T (tw) and the sums S_z2, S_w2.  The P rows
+// of 128 SNPs at a time are staged in LDS by the whole workgroup (sP: 128
+// rows of KM doubles, zero past K and past the split) and read back by every
+// lane alike (LDS broadcast); the genotype byte of the next row is loaded one
+// row ahead.
+constexpr int GHQ_CH = 32;   // byte rows (x 4 SNPs) per staged chunk
+
 template <int KM, int K0, int K1, bool OWN>
 __device__ __forceinline__ void gh_q_loop(const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int K,
-                                          const double* __restrict__ P, const double (&q)[KM], int64_t i, bool live,
-                                          int64_t r0, int64_t r1, int64_t m_end, double* hx, double* hz,
+                                          const double* __restrict__ P, double* sP, const double (&q)[KM],
+                                          int64_t i, bool live, int64_t r0, int64_t r1, double* hx, double* hz,
                                           double* tw, double& sz2, double& sw2) {
-    // The P row of the next SNP (the same address in every lane) and the
-    // genotype byte of the next row are loaded one step ahead.
-    double pc[KM];
+    for (int64_t c0 = r0; c0 < r1; c0 += GHQ_CH) {
+        const int nr = (int)(r1 - c0 < GHQ_CH ? r1 - c0 : GHQ_CH);
+        __syncthreads();   // the previous chunk's rows are read by every wave
+        for (int e = threadIdx.x; e < 4 * GHQ_CH * KM; e += 256) {
+            const int sl = e / KM, k = e - sl * KM;
+            const int64_t m = 4 * c0 + sl;
+            sP[e] = (sl < 4 * nr && m < M && k < K) ? P[m * K + k] : 0.0;
+        }
+        __syncthreads();
+        uint32_t bn = live ? G[c0 * ldg + i] : 0xFFu;
+        for (int rr = 0; rr < nr; ++rr) {
+            const uint32_t byte = bn;
+            if (rr + 1 < nr) bn = live ? G[(c0 + rr + 1) * ldg + i] : 0xFFu;
+#pragma unroll 1
+            for (int f = 0; f < 4; ++f) {
+                // SNPs past M read as missing (code 3) and have zero P rows
+                const uint32_t code = 4 * (c0 + rr) + f < M ? (byte >> (2 * f)) & 3u : 3u;
+                const double* pr = sP + (4 * rr + f) * KM;
+                double pc[KM];
 #pragma unroll
-    for (int k = 0; k < KM; ++k) pc[k] = k < K ? P[4 * r0 * K + k] : 0.0;
-    __builtin_amdgcn_s_waitcnt(0xC07F);
-    uint32_t bn = live ? G[r0 * ldg + i] : 0xFFu;
-    for (int64_t r = r0; r < r1; ++r) {
-        const uint32_t byte = bn;
-        if (r + 1 < r1) bn = live ? G[(r + 1) * ldg + i] : 0xFFu;
+                for (int k = 0; k < KM; ++k) pc[k] = pr[k];
+                double h = 0.0;
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int64_t m = 4 * r + f;
-            if (m >= M) break;
-            const int64_t mn = m + 1 < m_end ? m + 1 : m;
-            // This row's scalar loads (issued one SNP ago) are complete
-            // before the next row's are issued: scalar loads return out of
-            // order, so a wait placed at the first use would also wait for
-            // the prefetch.  lgkmcnt(0), other counters left at their maximum.
-            __builtin_amdgcn_s_waitcnt(0xC07F);
-            double pn[KM];
+                for (int k = 0; k < KM; ++k) h += q[k] * pc[k];
+                const Weights w = bq_weights(h, code);
+                const double ws = w.w1 + w.w2;
+                if (OWN) {
+                    const double zd = w.z1 - w.z2;
+                    sz2 += w.z2;
+                    sw2 += w.w2;
 #pragma unroll
-            for (int k = 0; k < KM; ++k) pn[k] = P[mn * K + (k < K ? k : 0)];   // masked on use
-            // keep the scheduler from sinking those loads to their use
-            __builtin_amdgcn_sched_barrier(0);
-            const uint32_t code = (byte >> (2 * f)) & 3u;
-            double h = 0.0;
+                    for (int k = 0; k < KM; ++k) {
+                        hz[k] += zd * pc[k];
+                        tw[k] += w.w2 * pc[k];
+                    }
+                }
+                int t = 0;
 #pragma unroll
-            for (int k = 0; k < KM; ++k) h += q[k] * pc[k];
-            const Weights w = bq_weights(h, code);
-            const double ws = w.w1 + w.w2;
-            if (OWN) {
-                const double zd = w.z1 - w.z2;
-                sz2 += w.z2;
-                sw2 += w.w2;
+                for (int k = K0; k < K1; ++k) {
+                    const double a = ws * pc[k];
 #pragma unroll
-                for (int k = 0; k < KM; ++k) {
-                    hz[k] += zd * pc[k];
-                    tw[k] += w.w2 * pc[k];
+                    for (int l = k; l < KM; ++l, ++t) hx[t] += a * pc[l];
                 }
             }
-            int t = 0;
-#pragma unroll
-            for (int k = K0; k < K1; ++k) {
-                const double a = ws * pc[k];
-#pragma unroll
-                for (int l = k; l < KM; ++l, ++t) hx[t] += a * pc[l];
-            }
-#pragma unroll
-            for (int k = 0; k < KM; ++k) pc[k] = k < K ? pn[k] : 0.0;
         }
     }
 }
@@ -165,6 +165,7 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
     constexpr int PB = KS > 0 ? tri_pairs(KM, KS, KM) : 0;
     constexpr int PX = PA > PB ? PA : PB;
     __shared__ double s_corr[KS > 0 ? 128 : 1][KM + 1];
+    __shared__ __attribute__((aligned(16))) double sP[4 * GHQ_CH * KM];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int role = KS > 0 ? (wave & 1) : 0;
     const int slot = KS > 0 ? (wave >> 1) * 64 + lane : 0;
@@ -172,8 +173,7 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
     const int64_t Mb = (M + 3) >> 2;
     const int64_t r0 = (int64_t)blockIdx.y * rows_per_split;
     const int64_t r1 = r0 + rows_per_split < Mb ? r0 + rows_per_split : Mb;
-    const int64_t m_end = 4 * r1 < M ? 4 * r1 : M;     // SNPs of this split: 4 r0 .. m_end - 1
-    if (4 * r0 >= m_end) return;                        // the whole workgroup
+    if (r0 >= r1) return;                               // the whole workgroup
     const bool live = i < N;
     double q[KM], hx[PX], hz[KM], tw[KM];
     double sz2 = 0.0, sw2 = 0.0;
@@ -186,15 +186,15 @@ __global__ __launch_bounds__(256) void grad_hess_q_kernel(const uint8_t* __restr
 #pragma unroll
     for (int t = 0; t < PX; ++t) hx[t] = 0.0;
     if (KS == 0) {
-        gh_q_loop<KM, 0, KM, true>(G, ldg, M, K, P, q, i, live, r0, r1, m_end, hx, hz, tw, sz2, sw2);
+        gh_q_loop<KM, 0, KM, true>(G, ldg, M, K, P, sP, q, i, live, r0, r1, hx, hz, tw, sz2, sw2);
         if (live) gh_q_store<KM, 0, KM, true>(K, i, hx, hz, tw, sz2, sw2, XtX, Xtz);
         return;
     }
     constexpr int KA = KS > 0 ? KS : KM;
     if (role == 0)
-        gh_q_loop<KM, 0, KA, true>(G, ldg, M, K, P, q, i, live, r0, r1, m_end, hx, hz, tw, sz2, sw2);
+        gh_q_loop<KM, 0, KA, true>(G, ldg, M, K, P, sP, q, i, live, r0, r1, hx, hz, tw, sz2, sw2);
     else
-        gh_q_loop<KM, KA, KM, false>(G, ldg, M, K, P, q, i, live, r0, r1, m_end, hx, hz, tw, sz2, sw2);
+        gh_q_loop<KM, KA, KM, false>(G, ldg, M, K, P, sP, q, i, live, r0, r1, hx, hz, tw, sz2, sw2);
     if (role == 0) {
 #pragma unroll
         for (int k = 0; k < KM; ++k) s_corr[slot][k] = tw[k];
