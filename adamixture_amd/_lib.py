@@ -84,8 +84,8 @@ SIGNATURES = {
     "adx_select_snps": (_int, [_p, _i64, _p, _i64, _i64, _p, _p]),
     "adx_gemm_tn_f64": (_int, [_p, _i64, _p, _i64, _i64, _int, _int, _dbl, _p, _i64, _p, _p]),
     "adx_gemm_nn_f64": (_int, [_p, _i64, _i64, _int, _int, _p, _i64, _dbl, _p, _p, _dbl, _p, _i64, _p]),
-    "adx_set_em_tiles": (_int, [_int, _int, _i64]),
-    "adx_get_em_tiles": (_int, [_int, _p, _p]),
+    "adx_set_em_tiles": (_int, [_int, _int, _int, _i64]),
+    "adx_get_em_tiles": (_int, [_int, _p, _p, _p]),
     "adx_mlp_gemm_f32": (_int, [_int, _int, _i64, _i64, _i64, _flt, _p, _i64, _p, _i64, _flt, _p, _i64, _int, _p]),
     "adx_mlp_bias_act": (_int, [_p, _p, _i64, _i64, _i64, _int, _p]),
     "adx_mlp_relu_bwd": (_int, [_p, _p, _i64, _p]),

@@ -1,11 +1,14 @@
 """Tile-shape autotuner for the EM passes (project extension; the reference
 has fixed launch shapes, adamixture/src/utils_c/cuda/*.cu).
 
-The EM kernels take two launch parameters, read by libadx at every launch
+The EM kernels take three launch parameters, read by libadx at every launch
 (adx_set_em_tiles): ``cw``, the number of 16-SNP column tiles a wavefront
-owns (register footprint vs. reuse of the sample-side operands), and
-``target_wgs``, the number of workgroups the sample range is split to reach
-(grid fill vs. atomic traffic of the sample-side reduction).  ``tune_em``
+owns (register footprint vs. reuse of the sample-side operands); ``waves``,
+the wavefronts per workgroup (4 or 8: a workgroup owns 16 cw waves SNPs and
+writes one partial-T slice for them, so 8 waves halve that traffic at the
+same occupancy); and ``target_wgs``, the number of workgroups the sample
+range is split to reach (grid fill vs. the A/B partial slices of the
+sample-side reduction).  ``tune_em``
 times each candidate on the caller's own genotypes with HIP events on the
 launch stream and keeps the fastest; results are cached per (device,
 precision, M, N, K) in memory and, if ``ADX_TUNE_CACHE`` names a file, on
@@ -29,21 +32,28 @@ log = logging.getLogger(__name__)
 DTYPE_TAG = {"f32": _lib.ADX_F32, "f64": _lib.ADX_F64, "bf16": _lib.ADX_BF16}
 TARGETS = (1024, 2048, 4096, 8192)
 CANDIDATES = {
-    "f32": [(cw, t) for cw in (4, 8) for t in TARGETS],
-    "f64": [(4, t) for t in TARGETS],
-    "bf16": [(cw, t) for cw in (4, 8) for t in TARGETS],
+    "f32": [(cw, wv, t) for cw in (4, 8) for wv in (4, 8) for t in TARGETS],
+    "f64": [(4, wv, t) for wv in (4, 8) for t in TARGETS],
+    "bf16": [(cw, wv, t) for cw in (4, 8) for wv in (4, 8) for t in TARGETS],
 }
-_MEM: dict[str, tuple[int, int]] = {}
+_MEM: dict[str, tuple[int, int, int]] = {}
 
 
-def get_tiles(precision: str) -> tuple[int, int]:
-    cw, tg = ctypes.c_int(0), ctypes.c_int64(0)
-    _lib.call("adx_get_em_tiles", DTYPE_TAG[precision], ctypes.addressof(cw), ctypes.addressof(tg))
-    return cw.value, tg.value
+def get_tiles(precision: str) -> tuple[int, int, int]:
+    """The current (cw, waves, target_wgs) of ``precision``."""
+    cw, wv, tg = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int64(0)
+    _lib.call("adx_get_em_tiles", DTYPE_TAG[precision], ctypes.addressof(cw), ctypes.addressof(wv),
+              ctypes.addressof(tg))
+    return cw.value, wv.value, tg.value
 
 
-def set_tiles(precision: str, cw: int, target_wgs: int) -> None:
-    _lib.call("adx_set_em_tiles", DTYPE_TAG[precision], int(cw), int(target_wgs))
+def set_tiles(precision: str, cw: int, waves: int, target_wgs: int) -> None:
+    _lib.call("adx_set_em_tiles", DTYPE_TAG[precision], int(cw), int(waves), int(target_wgs))
+
+
+def tiles_label(tiles) -> str:
+    """'cw/waves/target_wgs', the key of a tile shape in timing tables."""
+    return "/".join(str(int(x)) for x in tiles)
 
 
 def _key(device, precision: str, M: int, N: int, K: int) -> str:
@@ -95,7 +105,7 @@ def _stats_pass(G: PackedGenotypes, P, Q, precision: str, bufs: dict) -> None:
 def tune_em(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str, *, repeats: int = 2,
             candidates=None) -> dict:
     """Time every candidate tile shape on (G, P, Q); set and return the best.
-    Returns {"best": (cw, target_wgs), "ms": {"cw/target": ms, ...}}."""
+    Returns {"best": (cw, waves, target_wgs), "ms": {"cw/waves/target": ms, ...}}."""
     M, K = P.shape
     N = Q.shape[0]
     dev = G.device
@@ -106,8 +116,8 @@ def tune_em(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str
     cands = candidates or CANDIDATES[precision]
     if precision == "bf16":
         nb = 0                                      # the operand buffer holds per-tile partial sums
-        for cw, tg in cands:
-            set_tiles(precision, cw, tg)
+        for tiles in cands:
+            set_tiles(precision, *tiles)
             nb = max(nb, _lib.load().adx_em_bf16_operand_bytes(M, N, K, None))
         set_tiles(precision, *before)
         bufs["ops"] = torch.zeros(nb, dtype=torch.uint8, device=dev)
@@ -117,8 +127,8 @@ def tune_em(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str
         bufs["P32"] = P.to(torch.float32).contiguous()
         bufs["Q32"] = Q.to(torch.float32).contiguous()
     timings = {}
-    for cw, tg in cands:
-        set_tiles(precision, cw, tg)
+    for tiles in cands:
+        set_tiles(precision, *tiles)
         _stats_pass(G, P, Q, precision, bufs)          # warm-up (code load, caches)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -126,21 +136,23 @@ def tune_em(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str
             _stats_pass(G, P, Q, precision, bufs)
         e1.record()
         e1.synchronize()
-        timings[f"{cw}/{tg}"] = e0.elapsed_time(e1) / repeats
+        timings[tiles_label(tiles)] = e0.elapsed_time(e1) / repeats
     best = min(timings, key=timings.get)
-    cw, tg = (int(x) for x in best.split("/"))
-    set_tiles(precision, cw, tg)
-    log.info(f"    autotune {precision} M={M} N={N} K={K}: best cw={cw} target_wgs={tg} "
-             f"({timings[best]:.3f} ms; default {before[0]}/{before[1]}: "
-             f"{timings.get(f'{before[0]}/{before[1]}', float('nan')):.3f} ms)")
-    return {"best": (cw, tg), "ms": timings, "default": before}
+    cw, wv, tg = (int(x) for x in best.split("/"))
+    set_tiles(precision, cw, wv, tg)
+    log.info(f"    autotune {precision} M={M} N={N} K={K}: best cw={cw} waves={wv} target_wgs={tg} "
+             f"({timings[best]:.3f} ms; default {tiles_label(before)}: "
+             f"{timings.get(tiles_label(before), float('nan')):.3f} ms)")
+    return {"best": (cw, wv, tg), "ms": timings, "default": before}
 
 
-def ensure_tuned(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str) -> tuple[int, int]:
+def ensure_tuned(G: PackedGenotypes, P: torch.Tensor, Q: torch.Tensor, precision: str) -> tuple[int, int, int]:
     """Apply the cached tile shape for this problem, tuning it on first use."""
     M, K = P.shape
     key = _key(G.device, precision, M, Q.shape[0], K)
     hit = _MEM.get(key) or (tuple(_load_disk()[key]) if key in _load_disk() else None)
+    if hit is not None and len(hit) != 3:      # an entry from before the waves parameter
+        hit = None
     if hit is None:
         hit = tune_em(G, P, Q, precision)["best"]
         _store_disk(key, list(hit))

@@ -34,14 +34,30 @@ int fail_msg(int code, const std::string& msg);
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Tile configuration of the EM passes for one dtype tag (ADX_F32, ADX_F64,
-// ADX_BF16): SNP column tiles per wave and the target number of workgroups
-// (the sample range is split until the grid reaches it).  Process-wide, set
-// by adx_set_em_tiles (the autotuner); read at every launch.
+// ADX_BF16): 16-SNP column tiles per wave, waves per workgroup (4 or 8; a
+// workgroup owns 16 * cw * waves SNPs and writes one partial T slice per
+// sample for them), and the target number of workgroups (the sample range
+// is split until the grid reaches it).  Process-wide, set by
+// adx_set_em_tiles (the autotuner); read at every launch.
 struct EmTiles {
     int cw;
+    int waves;
     int64_t target_wgs;
 };
 EmTiles em_tiles(int dtype);
+
+// Sum of the WV per-wave LDS slices s[0..WV-1][e] in a fixed order: adjacent
+// pairs first, ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)).
+template <int WV, typename T, int LEN>
+__device__ __forceinline__ T sum_wave_slices(const T (*s)[LEN], int e) {
+    static_assert(WV == 4 || WV == 8, "4 or 8 waves per workgroup");
+    const T a = (s[0][e] + s[1][e]) + (s[2][e] + s[3][e]);
+    if constexpr (WV == 4) {
+        return a;
+    } else {
+        return a + ((s[4][e] + s[5][e]) + (s[6][e] + s[7][e]));
+    }
+}
 
 // One 16x16x4 MFMA step.  Operand layouts are shared by FP32 and FP64:
 //   A (16x4):  lane l supplies A[l % 16][l / 16]

@@ -78,8 +78,8 @@ __device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double
     b = code == 3u ? 0.0 : (2.0 - g) * (hc * r);
 }
 
-template <typename T, int KS, int CW, bool STATS, bool LL>
-__global__ __launch_bounds__(256) void em_pass_kernel(
+template <typename T, int KS, int CW, int WV, bool STATS, bool LL>
+__global__ __launch_bounds__(64 * WV) void em_pass_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
     int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
     T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart,
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
     const int cl = lane & 15;
-    const int64_t snp_w = ((int64_t)blockIdx.x * 4 + wave) * (16 * CW);
+    const int64_t snp_w = ((int64_t)blockIdx.x * WV + wave) * (16 * CW);
     const int64_t row_w = snp_w >> 2;
     const int64_t Mb = (M + 3) >> 2;
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
@@ -105,14 +105,14 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
     // unpadded 16-element stride puts the 16 lanes of a column write on one
     // bank.
     constexpr int TSX = 17;
-    __shared__ T s_tile[4][16 * TSX];
+    __shared__ T s_tile[WV][16 * TSX];
     // Per wave: T of the 64 samples of the current row block over the wave's
     // SNPs.  Each wave writes each (sample, k) once per row block, so the
     // slices need no atomics and are summed in a fixed order at the flush.
     // Rows padded to 17 elements: the 64 lanes of a write (4 lane groups x
     // 16 columns, rows 16 apart) fall on distinct banks.
-    __shared__ T s_tsum[STATS ? 4 : 1][64 * 17];
-    __shared__ double s_ll[4];
+    __shared__ T s_tsum[STATS ? WV : 1][64 * 17];
+    __shared__ double s_ll[WV][1];
 
     // P operands, constant for the whole strip.
     T pH[CW][KS];
@@ -236,18 +236,18 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
         }
 
         if (STATS) {
-            // T of these 64 samples over this workgroup's SNPs (the four wave
+            // T of these 64 samples over this workgroup's SNPs (the WV wave
             // slices summed in a fixed order): a plain store into the
             // workgroup's slice of Tpart (each (slice, sample) is written
             // once; tpart_reduce_kernel sums the slices), or, without a
             // workspace, a device-scope atomic (performed at the memory side).
             __syncthreads();
-            for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+            for (int e = threadIdx.x; e < 64 * 16; e += 64 * WV) {
                 const int kp = e & 15;
                 const int64_t smp = s0 + (e >> 4);
                 const int ex = (e >> 4) * 17 + kp;
                 if (kb0 + kp < K && smp < N) {
-                    const T v = (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
+                    const T v = sum_wave_slices<WV>(s_tsum, ex);
                     if (Tpart)
                         Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] = v;
                     else
@@ -289,10 +289,9 @@ __global__ __launch_bounds__(256) void em_pass_kernel(
         double ll = lp.log_value();
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) ll += __shfl_xor(ll, off);
-        if (lane == 0) s_ll[wave] = ll;
+        if (lane == 0) s_ll[wave][0] = ll;
         __syncthreads();
-        if (threadIdx.x == 0)
-            ll_part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = (s_ll[0] + s_ll[1]) + (s_ll[2] + s_ll[3]);
+        if (threadIdx.x == 0) ll_part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sum_wave_slices<WV>(s_ll, 0);
     }
 }
 
@@ -434,12 +433,12 @@ __global__ void tpart_reduce_kernel(const T* __restrict__ Tpart, int64_t nparts,
 // Host-side dispatch
 // ---------------------------------------------------------------------------
 // Defaults: the fastest shapes measured at N=100,000 x M=500,000, K=10.
-static EmTiles g_tiles[3] = {{8, 8192}, {4, 4096}, {8, 8192}};  // ADX_F32, ADX_F64, ADX_BF16
+static EmTiles g_tiles[3] = {{8, 4, 8192}, {4, 4, 4096}, {8, 4, 8192}};  // ADX_F32, ADX_F64, ADX_BF16
 
 EmTiles em_tiles(int dtype) { return g_tiles[dtype >= 0 && dtype < 3 ? dtype : 0]; }
 
 struct EmGrid {
-    int cw;
+    int cw, waves;
     int64_t nbx, splits, n_per_split;
 };
 
@@ -448,11 +447,20 @@ struct EmGrid {
 // first only), at CW = 4.
 static bool em_wide(int K) { return K > 16; }
 
-static EmGrid em_grid(int dtype, int64_t M, int64_t N, int K, int64_t target = 0) {
+// Waves per workgroup of a launch: the tiles' choice for the statistics
+// pass with K <= 16; 4 for the wide form and for every pass that carries
+// the log-likelihood (the fused FP64 statistics + log-likelihood kernel
+// needs more than 256 registers per lane, i.e. one wave per SIMD).
+static int em_waves(int dtype, int K, bool stats, bool ll) {
+    return (stats && !ll && !em_wide(K)) ? em_tiles(dtype).waves : 4;
+}
+
+static EmGrid em_grid(int dtype, int64_t M, int64_t N, int K, int waves, int64_t target = 0) {
     const EmTiles tl = em_tiles(dtype);
     EmGrid gd;
     gd.cw = em_wide(K) ? 4 : tl.cw;
-    gd.nbx = ceil_div(M, 64 * gd.cw);
+    gd.waves = waves;
+    gd.nbx = ceil_div(M, 16 * gd.cw * gd.waves);
     if (target <= 0) target = tl.target_wgs;
     int64_t s = ceil_div(target, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
@@ -469,16 +477,32 @@ static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_
                       int K, const T* P, int64_t ldp, const T* Q, int64_t ldq, T* A, T* B, T* Tm,
                       double* llp, T* Tpart, T* ABpart, int kb0) {
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
-    if constexpr (sizeof(T) == 4 && KS <= 4) {  // the FP64 and wide forms have a single tile width
-        if (gd.cw == 8) {
-            em_pass_kernel<T, KS, 8, STATS, LL>
-                <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart,
-                                       ABpart, kb0);
+#define ADX_EM_LAUNCH(cw, wv)                                                                                   \
+    em_pass_kernel<T, KS, cw, wv, STATS, LL><<<grid, 64 * wv, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq,        \
+                                                                       gd.n_per_split, A, B, Tm, llp, Tpart, \
+                                                                       ABpart, kb0)
+    // The FP64 and wide forms have a single tile width; 8-wave workgroups
+    // exist for the statistics-only pass with K <= 16 (em_waves).
+    if constexpr (STATS && !LL && KS <= 4) {
+        if (gd.waves == 8) {
+            if constexpr (sizeof(T) == 4) {
+                if (gd.cw == 8) {
+                    ADX_EM_LAUNCH(8, 8);
+                    return;
+                }
+            }
+            ADX_EM_LAUNCH(4, 8);
             return;
         }
     }
-    em_pass_kernel<T, KS, 4, STATS, LL>
-        <<<grid, 256, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq, gd.n_per_split, A, B, Tm, llp, Tpart, ABpart, kb0);
+    if constexpr (sizeof(T) == 4 && KS <= 4) {
+        if (gd.cw == 8) {
+            ADX_EM_LAUNCH(8, 4);
+            return;
+        }
+    }
+    ADX_EM_LAUNCH(4, 4);
+#undef ADX_EM_LAUNCH
 }
 
 // One launch per 16-wide k block (one for K <= 16); the log-likelihood
@@ -521,7 +545,7 @@ static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int6
 }
 
 // A pass over SNP bands.  Band b covers SNPs b*band .. min((b+1)*band, M) - 1
-// (band a multiple of the 64*cw SNPs of one workgroup column) and is one
+// (band a multiple of the 16*cw*waves SNPs of one workgroup column) and is one
 // launch sequence over the sub-matrix of its byte rows; its T / A / B
 // partial slices live in one region of the workspace reused band after band,
 // and its log-likelihood partials are summed into one slot per band.  The
@@ -529,6 +553,7 @@ static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int6
 // pass is deterministic.  Workspace layout:
 //   [LL partials of one band][LL band slots][T slices | A slices | B slices]
 struct EmPlan {
+    int waves;             // waves per workgroup of every launch (em_waves)
     int64_t band, nbands, target;
     int64_t ll_slots;      // LL partials of the largest band (its nbx x splits)
     int64_t ll_bytes;      // both LL regions, 256-byte padded
@@ -537,13 +562,16 @@ struct EmPlan {
 
 static int64_t pad256(int64_t b) { return ceil_div(b, 256) * 256; }
 
-static EmPlan em_plan_for(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t band, int64_t target) {
+static EmPlan em_plan_for(int dtype, int64_t M, int64_t N, int K, bool stats, bool ll, int64_t band,
+                          int64_t target) {
     EmPlan p;
+    p.waves = em_waves(dtype, K, stats, ll);
     p.band = band < M ? band : M;
     p.nbands = ceil_div(M, p.band);
     p.target = target;
     const int64_t last = M - (p.nbands - 1) * p.band;
-    const EmGrid g1 = em_grid(dtype, p.band, N, K, target), g2 = em_grid(dtype, last, N, K, target);
+    const EmGrid g1 = em_grid(dtype, p.band, N, K, p.waves, target);
+    const EmGrid g2 = em_grid(dtype, last, N, K, p.waves, target);
     const int64_t s1 = g1.nbx * g1.splits, s2 = g2.nbx * g2.splits;
     p.ll_slots = s1 > s2 ? s1 : s2;
     p.ll_bytes = pad256(8 * p.ll_slots) + pad256(8 * p.nbands);
@@ -561,11 +589,11 @@ static int64_t em_plan_bytes(const EmPlan& p) { return p.ll_bytes + p.slice_byte
 // splits, fewer A / B slices) that fits.
 // False if none does.
 // max_band > 0 caps the band (SNPs; the host-streamed pass's staging).
-static bool em_plan(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t ws, EmPlan& out,
+static bool em_plan(int dtype, int64_t M, int64_t N, int K, bool stats, bool ll, int64_t ws, EmPlan& out,
                     int64_t max_band = 0) {
     const EmTiles tl = em_tiles(dtype);
-    const int64_t unit = 64 * (em_wide(K) ? 4 : tl.cw);
-    out = em_plan_for(dtype, M, N, K, stats, M, tl.target_wgs);
+    const int64_t unit = 16 * (em_wide(K) ? 4 : tl.cw) * em_waves(dtype, K, stats, ll);
+    out = em_plan_for(dtype, M, N, K, stats, ll, M, tl.target_wgs);
     if (em_plan_bytes(out) <= ws && (max_band <= 0 || M <= max_band)) return true;
     int64_t cmax = ceil_div(M, unit);
     if (max_band > 0 && max_band / unit < cmax) cmax = max_band / unit;
@@ -575,13 +603,13 @@ static bool em_plan(int dtype, int64_t M, int64_t N, int K, bool stats, int64_t 
         int64_t lo = 0, hi = cmax + 1;   // fits(lo) (or lo = 0), !fits(hi) (or hi = cmax + 1)
         while (hi - lo > 1) {
             const int64_t c = (lo + hi) / 2;
-            if (em_plan_bytes(em_plan_for(dtype, M, N, K, stats, c * unit, target)) <= ws)
+            if (em_plan_bytes(em_plan_for(dtype, M, N, K, stats, ll, c * unit, target)) <= ws)
                 lo = c;
             else
                 hi = c;
         }
         if (lo >= 1) {
-            out = em_plan_for(dtype, M, N, K, stats, lo * unit, target);
+            out = em_plan_for(dtype, M, N, K, stats, ll, lo * unit, target);
             if (em_plan_bytes(out) <= ws) return true;
         }
     }
@@ -648,14 +676,14 @@ static int em_pass_run(const char* name, int dtype, const uint8_t* G, int64_t ld
     const bool ll = flags & ADX_EM_LOGLIK;
     hipStream_t st = (hipStream_t)stream;
     EmPlan plan;
-    const bool planned = workspace && em_plan(dtype, M, N, K, stats, workspace_bytes, plan);
+    const bool planned = workspace && em_plan(dtype, M, N, K, stats, ll, workspace_bytes, plan);
     if (!planned) {
         if (ll || !allow_atomics)
             return fail_msg(1, who + ": ADX_EM_LOGLIK needs a workspace (adx_em_workspace_bytes, or enough "
                                      "for one SNP band)");
         // no workspace: T, A, B by device-scope atomics (order varies run to run)
-        launch_any(dtype, true, false, em_grid(dtype, M, N, K), st, G, ldg, 0, M, N, K, P, ldp, Q, ldq, A, B, T,
-                   nullptr, nullptr);
+        launch_any(dtype, true, false, em_grid(dtype, M, N, K, em_waves(dtype, K, true, false)), st, G, ldg, 0, M,
+                   N, K, P, ldp, Q, ldq, A, B, T, nullptr, nullptr);
         ADX_LAUNCH_CHECK(name);
         return 0;
     }
@@ -666,7 +694,7 @@ static int em_pass_run(const char* name, int dtype, const uint8_t* G, int64_t ld
     for (int64_t b = 0; b < plan.nbands; ++b) {
         const int64_t m0 = b * plan.band;
         const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
-        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.target);
+        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.waves, plan.target);
         launch_any(dtype, stats, ll, gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, T, llp, tpart);
         if (ll) reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, slots + b);
         ADX_LAUNCH_CHECK(name);
@@ -714,26 +742,33 @@ using namespace adx;
 
 extern "C" {
 
-int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs) {
+int adx_set_em_tiles(int dtype, int cw, int waves, int64_t target_wgs) {
     if (dtype < 0 || dtype > 2) return fail_msg(1, "adx_set_em_tiles: unknown dtype tag");
     const bool ok_cw = dtype == ADX_F64 ? cw == 4 : (cw == 4 || cw == 8);
     if (!ok_cw) return fail_msg(1, "adx_set_em_tiles: cw must be 4 or 8 (4 for ADX_F64)");
+    if (waves != 4 && waves != 8) return fail_msg(1, "adx_set_em_tiles: waves must be 4 or 8");
     if (target_wgs < 64 || target_wgs > 65536) return fail_msg(1, "adx_set_em_tiles: target_wgs not in [64, 65536]");
-    g_tiles[dtype] = EmTiles{cw, target_wgs};
+    g_tiles[dtype] = EmTiles{cw, waves, target_wgs};
     return 0;
 }
 
-int adx_get_em_tiles(int dtype, int* cw, int64_t* target_wgs) {
+int adx_get_em_tiles(int dtype, int* cw, int* waves, int64_t* target_wgs) {
     if (dtype < 0 || dtype > 2) return fail_msg(1, "adx_get_em_tiles: unknown dtype tag");
     if (cw) *cw = g_tiles[dtype].cw;
+    if (waves) *waves = g_tiles[dtype].waves;
     if (target_wgs) *target_wgs = g_tiles[dtype].target_wgs;
     return 0;
 }
 
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K) {
     if (M <= 0 || N <= 0) return 0;
-    // K = 0: the log-likelihood partials only
-    return em_plan_bytes(em_plan_for(dtype, M, N, K > 0 ? K : 1, K > 0, M, em_tiles(dtype).target_wgs));
+    // K = 0: the log-likelihood partials only.  With statistics, enough for
+    // one band both without and with the log-likelihood (their workgroup
+    // shapes may differ).
+    const int64_t tg = em_tiles(dtype).target_wgs;
+    const int64_t b0 = em_plan_bytes(em_plan_for(dtype, M, N, K > 0 ? K : 1, K > 0, false, M, tg));
+    const int64_t b1 = em_plan_bytes(em_plan_for(dtype, M, N, K > 0 ? K : 1, K > 0, true, M, tg));
+    return b0 > b1 ? b0 : b1;
 }
 
 int adx_em_pass(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K, const void* P,
@@ -747,7 +782,9 @@ int64_t adx_em_band_snps(int dtype, int64_t M, int64_t N, int K, int flags, int6
                           int64_t max_band) {
     if (M <= 0 || N <= 0 || K < 1 || K > 64 || (dtype != ADX_F32 && dtype != ADX_F64)) return -1;
     EmPlan plan;
-    if (!em_plan(dtype, M, N, K, (flags & ADX_EM_STATS) != 0, workspace_bytes, plan, max_band)) return -1;
+    if (!em_plan(dtype, M, N, K, (flags & ADX_EM_STATS) != 0, (flags & ADX_EM_LOGLIK) != 0, workspace_bytes, plan,
+                 max_band))
+        return -1;
     return plan.band;
 }
 
@@ -766,7 +803,7 @@ int adx_em_pass_host(int dtype, const uint8_t* G_host, int64_t ldg, int64_t M, i
     const int64_t half = (staging_bytes / 2) / 256 * 256;
     const int64_t max_band = (half / ldg) * 4;
     EmPlan plan;
-    if (!workspace || !em_plan(dtype, M, N, K, stats, workspace_bytes, plan, max_band))
+    if (!workspace || !em_plan(dtype, M, N, K, stats, ll, workspace_bytes, plan, max_band))
         return fail_msg(1, "adx_em_pass_host: the workspace and staging do not fit one SNP band "
                            "(adx_em_band_snps gives the band a workspace allows)");
     hipStream_t st = (hipStream_t)stream;
@@ -809,7 +846,7 @@ int adx_em_pass_host(int dtype, const uint8_t* G_host, int64_t ldg, int64_t M, i
             rc = fail(e, "adx_em_pass_host: staging copy");
             break;
         }
-        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.target);
+        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.waves, plan.target);
         // the band's rows sit at the start of buffer i: offset G by -m0 / 4 rows
         // inside launch_band's arithmetic, i.e. pass m0 for P, A, B only
         const void* Pb = static_cast<const uint8_t*>(P) + m0 * ldp * (dtype == ADX_F64 ? 8 : 4);
@@ -850,7 +887,8 @@ int adx_em_pass_noalloc(int dtype, const uint8_t* G, int64_t ldg, int64_t M, int
     const int bad = em_pass_check("adx_em_pass_noalloc", dtype, G, ldg, M, N, K, ldp, ldq, A, B, T, flags, ll_out);
     if (bad >= 0) return bad;
     EmPlan plan;
-    if (!em_plan(dtype, M, N, K, (flags & ADX_EM_STATS) != 0, ADX_EM_NOALLOC_SCRATCH_BYTES, plan))
+    if (!em_plan(dtype, M, N, K, (flags & ADX_EM_STATS) != 0, (flags & ADX_EM_LOGLIK) != 0,
+                 ADX_EM_NOALLOC_SCRATCH_BYTES, plan))
         return fail_msg(1, "adx_em_pass_noalloc: this shape needs more than the fixed scratch of "
                            "ADX_EM_NOALLOC_SCRATCH_BYTES for one SNP band (use adx_em_pass with a workspace)");
     void* ws = nullptr;

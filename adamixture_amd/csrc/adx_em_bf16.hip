@@ -70,8 +70,8 @@ __device__ __forceinline__ bf16x8 load_bf16x8(const __bf16* p) {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-template <int KH, int CW>
-__global__ __launch_bounds__(256) void em_pass_bf16_kernel(
+template <int KH, int CW, int WV>
+__global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
     const __bf16* __restrict__ Qx, const _Float16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
     float* __restrict__ Tpart, float* __restrict__ ABpart, int kb0, int kbt) {
@@ -82,19 +82,19 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
     const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
     const int cl = lane & 15;
-    const int64_t snp_w = ((int64_t)blockIdx.x * 4 + wave) * (16 * CW);
+    const int64_t snp_w = ((int64_t)blockIdx.x * WV + wave) * (16 * CW);
     const int64_t row_w = snp_w >> 2;
     const int64_t Mb = (M + 3) >> 2;
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
 
     // Per wave: word [sample row r][SNP column c] = f16 a' | f16 b' << 16.
-    __shared__ __attribute__((aligned(16))) uint32_t s_tile[4][16 * TS];
+    __shared__ __attribute__((aligned(16))) uint32_t s_tile[WV][16 * TS];
     // Per wave: T of the 64 samples of the current row block over the wave's
     // SNPs; each (sample, k) is written once per wave and row block, and the
-    // four slices are summed in a fixed order at the flush (deterministic).
+    // WV slices are summed in a fixed order at the flush (deterministic).
     // Rows padded to 17 floats so the 64 lanes of a write hit distinct banks.
-    __shared__ float s_tsum[4][64 * 17];
+    __shared__ float s_tsum[WV][64 * 17];
 
     const bf16x8 zero8 = bf16x8{};
     bf16x8 pH[CW][KH];
@@ -142,7 +142,8 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         grow[t] = G + (row < Mb ? row : Mb - 1) * ldg + 16 * g;
         gor[t] = row >= Mb ? 0xFFFFFFFFu : nf < 4 ? ((0xFFu << (2 * nf)) & 0xFFu) * 0x01010101u : 0u;
     }
-    // The Q operands of a 64-sample sub-tile are staged once per workgroup:
+    // The Q operands of a 64-sample sub-tile are staged once per workgroup by
+    // its first 256 threads (waves 0-3; the others skip the staging):
     // thread t copies Qt chunk (row t >> 3, samples 8 (t & 7) .. + 7) and KH
     // Qx chunks (16 bytes each) into LDS in lane-fragment order,
     //   s_qt[(2 tp + hl) 64 + lane]          = Qt row (hl ? 16 + cl : cl), samples 16g + 8tp .. + 7
@@ -167,15 +168,18 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         qx_dst[j] = ((((q >> 2) & 3) * KH + (ch >> 2)) * 64) + 16 * (ch & 3) + (((q >> 4) << 2) | (q & 3));
     }
     const u32x4 zero4 = u32x4{0, 0, 0, 0};
-    u32x4 qn_t, qn_x[KH];
+    const bool stager = WV == 4 || wave < 4;
+    u32x4 qn_t = zero4, qn_x[KH];
     uint32_t gn[CW / 4][4];
     auto load_next = [&](int64_t s) {
-        qn_t = *reinterpret_cast<const u32x4*>(qt_src + s);
+        if (stager) {
+            qn_t = *reinterpret_cast<const u32x4*>(qt_src + s);
 #pragma unroll
-        for (int j = 0; j < KH; ++j) {
-            const int64_t r = s + qx_row[j];
-            const u32x4 v = *reinterpret_cast<const u32x4*>(Qx + (r < N ? r : N - 1) * LX + qx_col[j]);
-            qn_x[j] = r < N ? v : zero4;
+            for (int j = 0; j < KH; ++j) {
+                const int64_t r = s + qx_row[j];
+                const u32x4 v = *reinterpret_cast<const u32x4*>(Qx + (r < N ? r : N - 1) * LX + qx_col[j]);
+                qn_x[j] = r < N ? v : zero4;
+            }
         }
 #pragma unroll
         for (int t = 0; t < CW / 4; ++t) {
@@ -187,9 +191,11 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         }
     };
     auto stage_next = [&](uint32_t (&gw)[CW / 4][4]) {
-        s_qt[qt_dst] = qn_t;
+        if (stager) {
+            s_qt[qt_dst] = qn_t;
 #pragma unroll
-        for (int j = 0; j < KH; ++j) s_qx[qx_dst[j]] = qn_x[j];
+            for (int j = 0; j < KH; ++j) s_qx[qx_dst[j]] = qn_x[j];
+        }
 #pragma unroll
         for (int t = 0; t < CW / 4; ++t)
 #pragma unroll
@@ -297,13 +303,12 @@ __global__ __launch_bounds__(256) void em_pass_bf16_kernel(
         // sample) is written exactly once), summed by tpart_reduce_kernel.
         // Device-scope float atomics would be performed at the memory side
         // (one 4-byte read-modify-write each, ~N K M / (64 CW) per pass).
-        for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+        for (int e = threadIdx.x; e < 64 * 16; e += 64 * WV) {
             const int kp = e & 15;
             const int64_t smp = s0 + (e >> 4);
             const int ex = (e >> 4) * 17 + kp;
             if (kb0 + kp < K && smp < N)
-                Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] =
-                    (s_tsum[0][ex] + s_tsum[1][ex]) + (s_tsum[2][ex] + s_tsum[3][ex]);
+                Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] = sum_wave_slices<WV>(s_tsum, ex);
         }
         __syncthreads();
     }
@@ -380,16 +385,19 @@ static int bf16_kh(int K) { return K <= 10 ? 1 : K <= 21 ? 2 : K <= 32 ? 3 : K <
 static int bf16_kb(int K) { return (K + 15) / 16; }
 
 struct Bf16Grid {
-    int cw;
+    int cw, waves;
     int64_t nbx, splits, n_per_split;
 };
 
 // K > 16: one launch per 16-wide k block at CW = 4 (H over the full K in each).
+// 8-wave workgroups for KH = 1 and, at CW = 4, KH = 2 (the KH = 2, CW = 8
+// kernel needs more than 256 registers per lane, one wave per SIMD).
 static Bf16Grid bf16_grid(int64_t M, int64_t N, int K) {
     const EmTiles tl = em_tiles(ADX_BF16);
     Bf16Grid gd;
     gd.cw = K > 16 ? 4 : tl.cw;
-    gd.nbx = ceil_div(M, 64 * gd.cw);
+    gd.waves = (K > 16 || (bf16_kh(K) == 2 && gd.cw == 8)) ? 4 : tl.waves;
+    gd.nbx = ceil_div(M, 16 * gd.cw * gd.waves);
     int64_t s = ceil_div(tl.target_wgs, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
     if (s > max_s) s = max_s;
@@ -425,19 +433,31 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
     dim3 grid((unsigned)gd.nbx, (unsigned)gd.splits);
     float* ABpart = Tpart + gd.nbx * N * (int64_t)K;
     const int kbt = bf16_kb(K);
+#define ADX_BF16_LAUNCH(cw, wv)                                                                               \
+    em_pass_bf16_kernel<KH, cw, wv><<<grid, 64 * wv, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, \
+                                                              Tpart, ABpart, 16 * kb, kbt)
     for (int kb = 0; kb < kbt; ++kb) {
         bool done = false;
         if constexpr (KH <= 2) {
-            if (gd.cw == 8) {
-                em_pass_bf16_kernel<KH, 8><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split,
-                                                                 Tpart, ABpart, 16 * kb, kbt);
+            if (gd.waves == 8) {
+                if constexpr (KH == 1) {
+                    if (gd.cw == 8) {
+                        ADX_BF16_LAUNCH(8, 8);
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    ADX_BF16_LAUNCH(4, 8);
+                    done = true;
+                }
+            } else if (gd.cw == 8) {
+                ADX_BF16_LAUNCH(8, 4);
                 done = true;
             }
         }
-        if (!done)
-            em_pass_bf16_kernel<KH, 4><<<grid, 256, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split,
-                                                             Tpart, ABpart, 16 * kb, kbt);
+        if (!done) ADX_BF16_LAUNCH(4, 4);
     }
+#undef ADX_BF16_LAUNCH
     // slices hold T / 8 (a' = a / 8) and 32 A, 32 B (Qt = 256 Q)
     const int64_t nk = N * K, mk = M * K;
     tpart_reduce_kernel<<<(unsigned)ceil_div(nk, 256), 256, 0, st>>>(Tpart, gd.nbx, nk, 8.0f, T);

@@ -62,7 +62,7 @@ def parse():
                     help="run one device copy of this many GB first (calibrates HBM counters under rocprofv3 --pmc)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default EM tile shape instead of timing the candidates first")
-    ap.add_argument("--tiles", default=None, help="fixed EM tile shape 'cw/target_wgs' (implies --no-autotune)")
+    ap.add_argument("--tiles", default=None, help="fixed EM tile shape 'cw/waves/target_wgs' (implies --no-autotune)")
     ap.add_argument("--no-ttt", action="store_true",
                     help="skip the time-to-target race (the headline; ~40 s per seed)")
     ap.add_argument("--ttt-seeds", default="42,43,44,45,46", help="initialisation seeds of the race (median reported)")
@@ -274,7 +274,7 @@ def main():
         ms = [a.elapsed_time(b) for a, b, ll in ev if not (prec == "f64" and ll)]
         kern_ms = float(np.mean(ms))
         rec = {"em_iters_per_s": round(args.steps / elapsed, 4), "ms_per_step": round(1e3 * elapsed / args.steps, 3),
-               "tiles": dict(zip(("cw", "target_wgs"), get_tiles(prec))),
+               "tiles": dict(zip(("cw", "waves", "target_wgs"), get_tiles(prec))),
                "roofline": roofline(prec, kern_ms, w)}
         if prec == "f64":
             ll_ms = [a.elapsed_time(b) for a, b, ll in ev if ll]
@@ -429,8 +429,8 @@ def time_to_target(G, w, dev, seeds, default_tiles):
     from adamixture_amd.genotypes import allele_frequencies
     from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP
     from adamixture_amd.learned_train import race
-    for prec, (cw, tg) in default_tiles.items():
-        set_tiles(prec, cw, tg)
+    for prec, tiles in default_tiles.items():
+        set_tiles(prec, *tiles)
     model = WarmStartMLP.load(dev) if WEIGHTS_PATH.exists() else None
     f = allele_frequencies(G)
     runs = []
@@ -440,7 +440,8 @@ def time_to_target(G, w, dev, seeds, default_tiles):
                          extra_plain=("f64",) if i == 0 else ()))
     out = {"seeds": list(seeds), "rel": runs[0]["rel"], "L_star_definition":
            "FP64 logL of the converged plain FP32 run (randomized SVD + ALS start) of the same seed",
-           "tiles": {k: {"cw": v[0], "target_wgs": v[1]} for k, v in default_tiles.items()}, "per_seed": runs}
+           "tiles": {k: dict(zip(("cw", "waves", "target_wgs"), v)) for k, v in default_tiles.items()},
+           "per_seed": runs}
     for form in ("plain_f32", "plain_bf16", "learned_bf16", "plain_f64"):
         recs = [r[form] for r in runs if form in r and r[form].get("seconds_to_target")]
         if not recs:

@@ -92,19 +92,23 @@ int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int
 /* Tile shapes of the EM passes (project autotuner, no counterpart in the
  * reference): for dtype tag ADX_F32 / ADX_F64 (adx_em_pass) or ADX_BF16
  * (adx_em_pass_bf16), cw = 16-SNP column tiles per wavefront (4 or 8; 4 for
- * ADX_F64) and target_wgs = workgroups the sample range is split to reach
- * (64..65536).  Process-wide and not synchronised between host threads
+ * ADX_F64), waves = wavefronts per workgroup (4 or 8: a workgroup owns
+ * 16 cw waves SNPs, and the partial-T slices shrink as that grows) and
+ * target_wgs = workgroups the sample range is split to reach (64..65536).
+ * waves applies to the statistics pass with K <= 16 (and, in the BF16 form,
+ * not to K > 10 at cw = 8); passes that carry the log-likelihood run 4-wave
+ * workgroups.  Process-wide and not synchronised between host threads
  * (set it before passes run on other threads); affects later launches and
  * adx_em_workspace_bytes (query the workspace after changing it).  Defaults:
- * F32 (8, 8192), F64 (4, 4096), BF16 (8, 8192). */
-int adx_set_em_tiles(int dtype, int cw, int64_t target_wgs);
-int adx_get_em_tiles(int dtype, int *cw, int64_t *target_wgs);
+ * F32 (8, 4, 8192), F64 (4, 4, 4096), BF16 (8, 4, 8192). */
+int adx_set_em_tiles(int dtype, int cw, int waves, int64_t target_wgs);
+int adx_get_em_tiles(int dtype, int *cw, int *waves, int64_t *target_wgs);
 
 /* Workspace (bytes) for adx_em_pass to run in one SNP band for the given
  * shape, dtype and current tile shape: the log-likelihood partial sums, the
- * per-workgroup partial sums of T (ceil(M / (64 cw)) x N x K of the pass
- * dtype) and the per-sample-split partial sums of A and B (2 x splits x M x
- * K).  K = 0 sizes the log-likelihood partials only.  Given less, the pass
+ * per-workgroup partial sums of T (ceil(M / (16 cw waves)) x N x K of the
+ * pass dtype) and the per-sample-split partial sums of A and B (2 x splits x
+ * M x K).  Enough for either flag set (with or without the log-likelihood).  K = 0 sizes the log-likelihood partials only.  Given less, the pass
  * runs in SNP bands (below). */
 int64_t adx_em_workspace_bytes(int dtype, int64_t M, int64_t N, int K);
 
@@ -187,7 +191,7 @@ int adx_em_pass_noalloc(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int
  * two FP16 terms).  Stated accuracy against the FP64 pass: 1e-3 relative on
  * A, B, T.
  * The operand buffer also holds the per-workgroup partial sums of T
- * (ceil(M / (64 cw)) x N x K FP32, cw from adx_set_em_tiles) and the
+ * (ceil(M / (16 cw waves)) x N x K FP32, cw and waves from adx_set_em_tiles) and the
  * per-sample-split partial sums of A and B (2 x splits x M x K), so its size
  * depends on the current tile shape: query adx_em_bf16_operand_bytes after
  * changing it.  The slices are summed in a fixed order: the pass is

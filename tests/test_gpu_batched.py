@@ -119,8 +119,10 @@ def test_autotune_picks_a_candidate_and_keeps_results(cuda, precision):
         print(f"autotune {precision}: {res}", flush=True)
         assert res["best"] in [tuple(c) for c in autotune.CANDIDATES[precision]]
         assert autotune.get_tiles(precision) == res["best"]
-        for cw, tg in autotune.CANDIDATES[precision][:3]:
-            autotune.set_tiles(precision, cw, tg)
+        cands = autotune.CANDIDATES[precision]
+        for tiles in cands[:2] + [c for c in cands if c[1] == 8][:2]:
+            print(f"case: {precision} tiles {autotune.tiles_label(tiles)}", flush=True)
+            autotune.set_tiles(precision, *tiles)
             PE, QE = opt.run_em_step(G, P, Q, G.M)
             tol = 1e-12 if precision == "f64" else 1e-4
             np.testing.assert_allclose(PE.cpu().numpy(), PE0.cpu().numpy(), rtol=tol, atol=tol)
@@ -128,7 +130,9 @@ def test_autotune_picks_a_candidate_and_keeps_results(cuda, precision):
     finally:
         autotune.set_tiles(precision, *default)
     with pytest.raises(RuntimeError):
-        autotune.set_tiles(precision, 5, 2048)
+        autotune.set_tiles(precision, 5, 4, 2048)
+    with pytest.raises(RuntimeError):
+        autotune.set_tiles(precision, 4, 6, 2048)
 
 
 def test_loglik_independent_of_tile_shape(cuda):
@@ -145,9 +149,9 @@ def test_loglik_independent_of_tile_shape(cuda):
     default = autotune.get_tiles("f64")
     vals = []
     try:
-        for cw, tg in autotune.CANDIDATES["f64"]:
-            print(f"case: logL tiles {cw}/{tg}", flush=True)
-            autotune.set_tiles("f64", cw, tg)
+        for tiles in autotune.CANDIDATES["f64"]:
+            print(f"case: logL tiles {autotune.tiles_label(tiles)}", flush=True)
+            autotune.set_tiles("f64", *tiles)
             vals.append(em_pass(G, P, Q, loglik=True, stats=False).item())
     finally:
         autotune.set_tiles("f64", *default)
