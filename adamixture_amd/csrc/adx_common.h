@@ -136,8 +136,10 @@ __device__ inline void project_simplex_row(const double* x, double* out, double*
 struct LogProd {
     double m = 1.0;
     int64_t e = 0;
+    // CLAMP = false: the caller knows the clamp is a no-op (H in range).
+    template <bool CLAMP = true>
     __device__ __forceinline__ void mul(double h, uint32_t code) {
-        const double hc = fmin(fmax(h, 1e-10), 1.0 - 1e-10);
+        const double hc = CLAMP ? fmin(fmax(h, 1e-10), 1.0 - 1e-10) : h;
         const double u = 1.0 - hc;
         const double f1 = code == 2u ? hc : u;
         const double f2 = code == 0u ? u : hc;

@@ -78,12 +78,19 @@ __device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double
     b = code == 3u ? 0.0 : (2.0 - g) * (hc * r);
 }
 
-template <typename T, int KS, int CW, int WV, bool STATS, bool LL>
-__global__ __launch_bounds__(64 * WV) void em_pass_kernel(
+// Row stride of the per-wave transpose tile (see em_pass_body).
+constexpr int TSX = 17;
+
+// The pass body.  LLFAST: the log-likelihood factors skip the clamp of H to
+// [1e-10, 1 - 1e-10], which ll_domain_kernel has shown to be a no-op for
+// this (P, Q) (LogProd::mul), saving three of the ~16 FP64 VALU
+// instructions per genotype.  The LDS arrays are the kernel's.
+template <typename T, int KS, int CW, int WV, bool STATS, bool LL, bool LLFAST>
+__device__ __forceinline__ void em_pass_body(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
     int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
     T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart,
-    T* __restrict__ ABpart, int kb0) {
+    T* __restrict__ ABpart, int kb0, T (*s_tile)[16 * TSX], T (*s_tsum)[64 * 17], double (*s_ll)[1]) {
     using MM = Mma16<T>;
     using acc_t = typename MM::acc;
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
@@ -98,21 +105,6 @@ __global__ __launch_bounds__(64 * WV) void em_pass_kernel(
     const int64_t n0 = (int64_t)blockIdx.y * n_per_split;
     const int64_t n1 = n0 + n_per_split < N ? n0 + n_per_split : N;
 
-    // Per-wave 16 x 16 transpose tile for the T contraction, rows padded to
-    // TSX = 17 elements: the column writes (lane (g, c) -> row c) and the row
-    // reads (lane (g, c) -> column c) then fall on distinct banks within each
-    // 16-lane (FP64) or 32-lane (FP32, at most 2-way) LDS pass, where an
-    // unpadded 16-element stride puts the 16 lanes of a column write on one
-    // bank.
-    constexpr int TSX = 17;
-    __shared__ T s_tile[WV][16 * TSX];
-    // Per wave: T of the 64 samples of the current row block over the wave's
-    // SNPs.  Each wave writes each (sample, k) once per row block, so the
-    // slices need no atomics and are summed in a fixed order at the flush.
-    // Rows padded to 17 elements: the 64 lanes of a write (4 lane groups x
-    // 16 columns, rows 16 apart) fall on distinct banks.
-    __shared__ T s_tsum[STATS ? WV : 1][64 * 17];
-    __shared__ double s_ll[WV][1];
 
     // P operands, constant for the whole strip.
     T pH[CW][KS];
@@ -200,7 +192,7 @@ __global__ __launch_bounds__(64 * WV) void em_pass_kernel(
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t code = (w >> (8 * j)) & 3u;
                     if (STATS) em_terms<T>(h[j], code, a[j], b[j]);
-                    if (LL) lp.mul((double)h[j], code);
+                    if (LL) lp.mul<!LLFAST>((double)h[j], code);
                 }
                 // 8 factors (two column blocks) between renormalisations: the
                 // mantissa stays >= 1e-160; the rescale is exact, so the result
@@ -293,6 +285,77 @@ __global__ __launch_bounds__(64 * WV) void em_pass_kernel(
         __syncthreads();
         if (threadIdx.x == 0) ll_part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sum_wave_slices<WV>(s_ll, 0);
     }
+}
+
+template <typename T, int KS, int CW, int WV, bool STATS, bool LL>
+__global__ __launch_bounds__(64 * WV) void em_pass_kernel(
+    const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
+    int64_t ldp, const T* __restrict__ Q, int64_t ldq, int64_t n_per_split, T* __restrict__ Aacc,
+    T* __restrict__ Bacc, T* __restrict__ Tacc, double* __restrict__ ll_part, T* __restrict__ Tpart,
+    T* __restrict__ ABpart, int kb0, const int* __restrict__ ll_fast) {
+    // Per-wave 16 x 16 transpose tile for the T contraction, rows padded to
+    // TSX = 17 elements: the column writes (lane (g, c) -> row c) and the row
+    // reads (lane (g, c) -> column c) then fall on distinct banks within each
+    // 16-lane (FP64) or 32-lane (FP32, at most 2-way) LDS pass, where an
+    // unpadded 16-element stride puts the 16 lanes of a column write on one
+    // bank.
+    __shared__ T s_tile[WV][16 * TSX];
+    // Per wave: T of the 64 samples of the current row block over the wave's
+    // SNPs.  Each wave writes each (sample, k) once per row block, so the
+    // slices need no atomics and are summed in a fixed order at the flush.
+    // Rows padded to 17 elements: the 64 lanes of a write (4 lane groups x
+    // 16 columns, rows 16 apart) fall on distinct banks.
+    __shared__ T s_tsum[STATS ? WV : 1][64 * 17];
+    __shared__ double s_ll[WV][1];
+    // ll_fast (written by ll_domain_kernel earlier on the stream, the same
+    // value for every thread): one uniform choice of the body per launch.
+    // The log-likelihood-only pass has both bodies; the fused statistics +
+    // log-likelihood kernel keeps the clamped one (hipcc 7.2 fails on the
+    // two inlined copies of that body).
+    if constexpr (LL && !STATS) {
+        if (ll_fast && *ll_fast) {
+            em_pass_body<T, KS, CW, WV, STATS, LL, true>(G, ldg, M, N, K, P, ldp, Q, ldq, n_per_split, Aacc, Bacc,
+                                                         Tacc, ll_part, Tpart, ABpart, kb0, s_tile, s_tsum, s_ll);
+            return;
+        }
+    }
+    em_pass_body<T, KS, CW, WV, STATS, LL, false>(G, ldg, M, N, K, P, ldp, Q, ldq, n_per_split, Aacc, Bacc, Tacc,
+                                                  ll_part, Tpart, ABpart, kb0, s_tile, s_tsum, s_ll);
+}
+
+// Whether clamping H = Q P^T to [1e-10, 1 - 1e-10] (the log-likelihood's
+// clamp, em_adam_gpu.py:238-246) is a no-op for this (P, Q): every P entry
+// in [2e-10, 1 - 2e-10], every Q entry >= 0 and every Q row sum within 1e-12
+// of 1.  Then each H is a convex combination of one SNP's P entries up to a
+// relative 1e-12 (plus FP64 rounding), so 1e-10 < H < 1 - 1e-10.  *flag
+// must be 1 on entry (ll_flag_init_kernel); any failing row (NaN included)
+// stores 0 (every writer stores the same value).  Grid-stride over rows.
+__global__ void ll_flag_init_kernel(int* flag) {
+    if (threadIdx.x == 0) *flag = 1;
+}
+
+__global__ void ll_domain_kernel(const double* __restrict__ P, int64_t M, int64_t ldp, const double* __restrict__ Q,
+                                 int64_t N, int64_t ldq, int K, int* __restrict__ flag) {
+    const double lo = 2e-10, hi = 1.0 - 2e-10;
+    bool ok = true;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M + N; r += stride) {
+        if (r < M) {
+            for (int k = 0; k < K; ++k) {
+                const double v = P[r * ldp + k];
+                ok = ok && (v >= lo && v <= hi);
+            }
+        } else {
+            double s = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const double v = Q[(r - M) * ldq + k];
+                ok = ok && (v >= 0.0);
+                s += v;
+            }
+            ok = ok && (fabs(s - 1.0) <= 1e-12);
+        }
+    }
+    if (!ok) *flag = 0;
 }
 
 // Deterministic sum of n FP64 partials (fixed order) into out[0].
@@ -440,6 +503,7 @@ EmTiles em_tiles(int dtype) { return g_tiles[dtype >= 0 && dtype < 3 ? dtype : 0
 struct EmGrid {
     int cw, waves;
     int64_t nbx, splits, n_per_split;
+    const int* ll_fast = nullptr;   // ll_domain_kernel's flag (log-likelihood passes)
 };
 
 // K > 16 ("wide"): the A/B/T contractions run in 16-wide k blocks, one
@@ -480,7 +544,7 @@ static void launch_em(const EmGrid& gd, hipStream_t st, const uint8_t* G, int64_
 #define ADX_EM_LAUNCH(cw, wv)                                                                                   \
     em_pass_kernel<T, KS, cw, wv, STATS, LL><<<grid, 64 * wv, 0, st>>>(G, ldg, M, N, K, P, ldp, Q, ldq,        \
                                                                        gd.n_per_split, A, B, Tm, llp, Tpart, \
-                                                                       ABpart, kb0)
+                                                                       ABpart, kb0, gd.ll_fast)
     // The FP64 and wide forms have a single tile width; 8-wave workgroups
     // exist for the statistics-only pass with K <= 16 (em_waves).
     if constexpr (STATS && !LL && KS <= 4) {
@@ -551,12 +615,12 @@ static void launch_em_k(const EmGrid& gd, hipStream_t st, const uint8_t* G, int6
 // and its log-likelihood partials are summed into one slot per band.  The
 // band slots are summed in band order at the end, so for a given plan the
 // pass is deterministic.  Workspace layout:
-//   [LL partials of one band][LL band slots][T slices | A slices | B slices]
+//   [LL partials of one band][LL band slots][domain flag][T slices | A slices | B slices]
 struct EmPlan {
     int waves;             // waves per workgroup of every launch (em_waves)
     int64_t band, nbands, target;
     int64_t ll_slots;      // LL partials of the largest band (its nbx x splits)
-    int64_t ll_bytes;      // both LL regions, 256-byte padded
+    int64_t ll_bytes;      // both LL regions and the domain flag, 256-byte padded
     int64_t slice_bytes;   // largest band's partial slices (0 without STATS)
 };
 
@@ -574,7 +638,7 @@ static EmPlan em_plan_for(int dtype, int64_t M, int64_t N, int K, bool stats, bo
     const EmGrid g2 = em_grid(dtype, last, N, K, p.waves, target);
     const int64_t s1 = g1.nbx * g1.splits, s2 = g2.nbx * g2.splits;
     p.ll_slots = s1 > s2 ? s1 : s2;
-    p.ll_bytes = pad256(8 * p.ll_slots) + pad256(8 * p.nbands);
+    p.ll_bytes = pad256(8 * p.ll_slots) + pad256(8 * p.nbands) + 256;
     const int64_t esz = dtype == ADX_F64 ? 8 : 4;
     const int64_t b1 = esz * K * (g1.nbx * N + 2 * g1.splits * p.band);
     const int64_t b2 = esz * K * (g2.nbx * N + 2 * g2.splits * last);
@@ -642,6 +706,18 @@ static void launch_any(int dtype, bool stats, bool ll, const EmGrid& gd, hipStre
         launch_band<double, false, true>(gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, Tm, llp, nullptr);
 }
 
+// Enqueue the log-likelihood domain check of (P, Q) (FP64) into the plan's
+// flag slot; returns the flag's device address for the passes that follow.
+static const int* ll_domain_check(hipStream_t st, const EmPlan& plan, uint8_t* ws, const void* P, int64_t M,
+                                  int64_t ldp, const void* Q, int64_t N, int64_t ldq, int K) {
+    int* flag = reinterpret_cast<int*>(ws + pad256(8 * plan.ll_slots) + pad256(8 * plan.nbands));
+    ll_flag_init_kernel<<<1, 64, 0, st>>>(flag);
+    const int64_t blocks = ceil_div(M + N, 256);
+    ll_domain_kernel<<<(unsigned)(blocks < 4096 ? blocks : 4096), 256, 0, st>>>(
+        static_cast<const double*>(P), M, ldp, static_cast<const double*>(Q), N, ldq, K, flag);
+    return flag;
+}
+
 // Argument checks of adx_em_pass (-1 if the call can go ahead; 0 if there is
 // nothing to do; else the failure status).
 static int em_pass_check(const char* name, int dtype, const uint8_t* G, int64_t ldg, int64_t M, int64_t N, int K,
@@ -691,10 +767,12 @@ static int em_pass_run(const char* name, int dtype, const uint8_t* G, int64_t ld
     double* llp = reinterpret_cast<double*>(ws);
     double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
     void* tpart = stats ? ws + plan.ll_bytes : nullptr;
+    const int* ll_fast = ll ? ll_domain_check(st, plan, ws, P, M, ldp, Q, N, ldq, K) : nullptr;
     for (int64_t b = 0; b < plan.nbands; ++b) {
         const int64_t m0 = b * plan.band;
         const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
-        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.waves, plan.target);
+        EmGrid gd = em_grid(dtype, Mb, N, K, plan.waves, plan.target);
+        gd.ll_fast = ll_fast;
         launch_any(dtype, stats, ll, gd, st, G, ldg, m0, Mb, N, K, P, ldp, Q, ldq, A, B, T, llp, tpart);
         if (ll) reduce_f64_kernel<<<1, 256, 0, st>>>(llp, gd.nbx * gd.splits, slots + b);
         ADX_LAUNCH_CHECK(name);
@@ -830,6 +908,7 @@ int adx_em_pass_host(int dtype, const uint8_t* G_host, int64_t ldg, int64_t M, i
     double* llp = reinterpret_cast<double*>(ws);
     double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
     void* tpart = stats ? ws + plan.ll_bytes : nullptr;
+    const int* ll_fast = (rc == 0 && ll) ? ll_domain_check(st, plan, ws, P, M, ldp, Q, N, ldq, K) : nullptr;
     for (int64_t b = 0; b < plan.nbands && rc == 0; ++b) {
         const int64_t m0 = b * plan.band;
         const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
@@ -846,7 +925,8 @@ int adx_em_pass_host(int dtype, const uint8_t* G_host, int64_t ldg, int64_t M, i
             rc = fail(e, "adx_em_pass_host: staging copy");
             break;
         }
-        const EmGrid gd = em_grid(dtype, Mb, N, K, plan.waves, plan.target);
+        EmGrid gd = em_grid(dtype, Mb, N, K, plan.waves, plan.target);
+        gd.ll_fast = ll_fast;
         // the band's rows sit at the start of buffer i: offset G by -m0 / 4 rows
         // inside launch_band's arithmetic, i.e. pass m0 for P, A, B only
         const void* Pb = static_cast<const uint8_t*>(P) + m0 * ldp * (dtype == ADX_F64 ? 8 : 4);
