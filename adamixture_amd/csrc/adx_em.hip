@@ -767,7 +767,8 @@ static int em_pass_run(const char* name, int dtype, const uint8_t* G, int64_t ld
     double* llp = reinterpret_cast<double*>(ws);
     double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
     void* tpart = stats ? ws + plan.ll_bytes : nullptr;
-    const int* ll_fast = ll ? ll_domain_check(st, plan, ws, P, M, ldp, Q, N, ldq, K) : nullptr;
+    // only the log-likelihood-only kernel reads the flag (em_pass_kernel)
+    const int* ll_fast = (ll && !stats) ? ll_domain_check(st, plan, ws, P, M, ldp, Q, N, ldq, K) : nullptr;
     for (int64_t b = 0; b < plan.nbands; ++b) {
         const int64_t m0 = b * plan.band;
         const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
@@ -908,7 +909,8 @@ int adx_em_pass_host(int dtype, const uint8_t* G_host, int64_t ldg, int64_t M, i
     double* llp = reinterpret_cast<double*>(ws);
     double* slots = reinterpret_cast<double*>(ws + pad256(8 * plan.ll_slots));
     void* tpart = stats ? ws + plan.ll_bytes : nullptr;
-    const int* ll_fast = (rc == 0 && ll) ? ll_domain_check(st, plan, ws, P, M, ldp, Q, N, ldq, K) : nullptr;
+    const int* ll_fast =
+        (rc == 0 && ll && !stats) ? ll_domain_check(st, plan, ws, P, M, ldp, Q, N, ldq, K) : nullptr;
     for (int64_t b = 0; b < plan.nbands && rc == 0; ++b) {
         const int64_t m0 = b * plan.band;
         const int64_t Mb = M - m0 < plan.band ? M - m0 : plan.band;
