@@ -52,6 +52,12 @@ K_MAX = D_OUT
 # adamixture/entry.py:68-69); an integer truncates it (the network was also
 # trained in that mode, which pays off only while ALS dominates the start).
 ALS_ITERS = None
+# The fallback gate compares the two starts' FP64 logL over the leading
+# SNPs only: ceil(M / GATE_FRACTION) of them, at least GATE_MIN_SNPS (all of
+# G when that is M or more), in whole byte rows.  At 100,000 x 500,000 the
+# two full passes cost 68 ms of a 0.3 s start; the band, 8.5 ms.
+GATE_FRACTION = 8
+GATE_MIN_SNPS = 20_000
 # (name, rows, cols) of the flat parameter vector, in order.
 BLOCKS = [("W1", D_IN, WIDTH), ("b1", 1, WIDTH), ("W2", WIDTH, WIDTH), ("b2", 1, WIDTH),
           ("W3", WIDTH, D_OUT), ("b3", 1, D_OUT)]
@@ -312,16 +318,36 @@ def load_meta(path=WEIGHTS_PATH) -> dict:
 # the warm start with its fallback
 # ---------------------------------------------------------------------------
 
+def gate_band(G):
+    """The genotypes the fallback gate reads: the leading
+    max(ceil(M / GATE_FRACTION), GATE_MIN_SNPS) SNPs in whole byte rows (a
+    view of G), or G itself when that covers M or G is host-streamed."""
+    from .genotypes import is_streamed
+    m = max(-(-G.M // GATE_FRACTION), GATE_MIN_SNPS)
+    rows = (m + 3) // 4
+    if is_streamed(G) or 4 * rows >= G.M:
+        return G
+    return PackedGenotypes(G.data[:rows], 4 * rows, G.N)
+
+
+def gate_loglik(G, P: torch.Tensor, Q: torch.Tensor) -> float:
+    """FP64 logL of (P, Q) over the gate's SNPs (gate_band)."""
+    from .em_adam_gpu import em_pass
+    Gb = gate_band(G)
+    return em_pass(Gb, P[: Gb.M].contiguous(), Q.contiguous(), loglik=True, stats=False).item()
+
+
 def learned_warm_start(G: PackedGenotypes, U2, S2, V2, f, K: int, model: WarmStartMLP, seed: int, *,
                        als_iters: int | None = ALS_ITERS, max_als: int = 1000, tol_als: float = 1e-4):
     """Warm start from the rank-2K SVD (U2, S2, V2): ALS on its first K
     components (to tolerance, or ``als_iters`` iterations), the network's Q,
     and P from Q by one decode-fused pass over G.  Returns
-    (P, Q, logL, used_learned).  If the learned start's FP64 logL does not
-    exceed that of the ALS start, the plain SVD + ALS start is returned
-    (ALS continued to its tolerance first when it was truncated)."""
+    (P, Q, gate_logL, used_learned), gate_logL being the returned start's
+    FP64 logL over the gate's SNPs (gate_band).  If the learned start's
+    gate logL does not exceed that of the ALS start, the plain SVD + ALS
+    start is returned (ALS continued to its tolerance first when it was
+    truncated)."""
     from .als_gpu import ALS_gpu
-    from .em_adam_gpu import em_pass
     t0 = time.time()
     dev = G.device
     Uk, Sk, Vk = U2[:, :K].contiguous(), S2[:K].contiguous(), V2[:, :K].contiguous()
@@ -330,10 +356,10 @@ def learned_warm_start(G: PackedGenotypes, U2, S2, V2, f, K: int, model: WarmSta
     X = features(U2, S2, V2, f, P_s, Q_s, K)
     Q = model.infer(X, K)
     P = p_from_q_data(G, f, Q)
-    ll = em_pass(G, P, Q, loglik=True, stats=False).item()
-    ll_s = em_pass(G, P_s.contiguous(), Q_s.contiguous(), loglik=True, stats=False).item()
+    ll = gate_loglik(G, P, Q)
+    ll_s = gate_loglik(G, P_s, Q_s)
     used = ll > ll_s
-    log.info(f"    Learned warm start logL {ll:.1f} vs SVD+ALS {ll_s:.1f} -> "
+    log.info(f"    Learned warm start gate logL {ll:.1f} vs SVD+ALS {ll_s:.1f} -> "
              f"{'learned' if used else 'fallback to SVD+ALS'} ({time.time() - t0:.3f}s)")
     if used:
         return P, Q, ll, True
@@ -341,4 +367,4 @@ def learned_warm_start(G: PackedGenotypes, U2, S2, V2, f, K: int, model: WarmSta
         return P_s, Q_s, ll_s, False
     P_a, Q_a = ALS_gpu(Uk, Sk, Vk, f, seed, G.M, K, max(max_als - als_iters, 0), tol_als, dev, P_init=P_s,
                        Q_init=Q_s)
-    return P_a, Q_a, em_pass(G, P_a.contiguous(), Q_a.contiguous(), loglik=True, stats=False).item(), False
+    return P_a, Q_a, gate_loglik(G, P_a, Q_a), False

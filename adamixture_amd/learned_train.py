@@ -274,8 +274,9 @@ def race(G, f, K: int, seed: int, device, model: WarmStartMLP | None, *, plain_p
     if model is not None:
         t3 = _sync_time(dev)
         U2, S2, V2 = _svd(G, G.N, G.M, f, 2 * K, seed, SVD_DEFAULTS["power"], SVD_DEFAULTS["tol_svd"], 0, dev)
-        P_w, Q_w, ll_w, used = learned_warm_start(G, U2, S2, V2, f, K, model, seed)
+        P_w, Q_w, _, used = learned_warm_start(G, U2, S2, V2, f, K, model, seed)
         t_init_l = _sync_time(dev) - t3
+        ll_w = em_pass(G, P_w, Q_w, loglik=True, stats=False).item()      # reported, untimed
         t4 = _sync_time(dev)
         P3, Q3, ch_l = run_to_stop(G, P_w, Q_w, K, dev, fast_precision, t4)
         summary(f"learned_{fast_precision}", t_init_l, ch_l, P3, Q3,

@@ -137,7 +137,7 @@ def c5(cuda):
 
 
 def test_c5_eval_learned_gate_and_forms(cuda, c5):
-    from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP, learned_warm_start
+    from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP, gate_band, gate_loglik, learned_warm_start
     from adamixture_amd.learned_train import plain_start
     G, w = c5
     K = w.K
@@ -147,15 +147,18 @@ def test_c5_eval_learned_gate_and_forms(cuda, c5):
     ll_a = em_pass(G, P_a, Q_a, loglik=True, stats=False).item()
     model = WarmStartMLP.load(cuda)
     assert WEIGHTS_PATH.exists()
-    P_l, Q_l, ll_l, used = learned_warm_start(G, U, S, V, f, K, model, w.seed)
-    print(f"C5: ALS start logL {ll_a:.1f}, learned start logL {ll_l:.1f}, gate used the network: {used}", flush=True)
-    # the gate returns the better start by FP64 logL, and a valid one
-    assert ll_l >= ll_a if used else ll_l == pytest.approx(ll_a, rel=1e-12)
+    P_l, Q_l, ll_gate, used = learned_warm_start(G, U, S, V, f, K, model, w.seed)
+    ll_l = em_pass(G, P_l, Q_l, loglik=True, stats=False).item()
+    print(f"C5: ALS start logL {ll_a:.1f}, learned start logL {ll_l:.1f}, gate used the network: {used} "
+          f"(gate band {gate_band(G).M} of {G.M} SNPs)", flush=True)
+    # the gate returns the better start by FP64 logL over its band, and a valid one
+    ll_a_gate = gate_loglik(G, P_a, Q_a)
+    assert ll_gate > ll_a_gate if used else ll_gate == pytest.approx(ll_a_gate, rel=1e-12)
     Ql = Q_l.cpu().numpy()
     # clamp to [1e-5, 1 - 1e-5] then rows divided by their sums (als_gpu.py:23-36):
     # an entry ends at least 1e-5 / (1 + K 1e-5)
     assert Ql.min() >= 1e-5 / (1 + K * 1e-5) * (1 - 1e-12) and np.allclose(Ql.sum(1), 1.0, atol=1e-12)
-    assert em_pass(G, P_l, Q_l, loglik=True, stats=False).item() == pytest.approx(ll_l, rel=1e-13)
+    assert gate_loglik(G, P_l, Q_l) == pytest.approx(ll_gate, rel=1e-13)
     # every form's EM statistics against the FP64 pass at the ALS start
     ref = EMAdamOptimizer(P_a.shape, Q_a.shape, 0.005, 0.8, 0.88, 1e-8, cuda, precision="f64")
     ref.accumulate(G, P_a, Q_a)

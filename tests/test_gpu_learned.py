@@ -11,7 +11,7 @@ import torch
 
 from adamixture_amd.em_adam_gpu import em_pass
 from adamixture_amd.genotypes import allele_frequencies
-from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP, features, learned_warm_start
+from adamixture_amd.learned_init import WEIGHTS_PATH, WarmStartMLP, features, gate_loglik, learned_warm_start
 from adamixture_amd.learned_train import align_columns, converge_plain, first_reach, plain_start, run_to_stop
 from adamixture_amd.synth import Workload, make_genotypes
 
@@ -31,7 +31,8 @@ def test_trained_network_beats_the_als_start(cuda):
     Q_w = model.infer(features(U, S, V, f, P_a, Q_a, K), K)
     mae_als = float((Q_a - Q_star).abs().sum(1).mean())
     mae_net = float((Q_w - Q_star).abs().sum(1).mean())
-    P_l, Q_l, ll_l, used = learned_warm_start(G, U, S, V, f, K, model, seed)
+    P_l, Q_l, ll_gate, used = learned_warm_start(G, U, S, V, f, K, model, seed)
+    ll_l = em_pass(G, P_l, Q_l, loglik=True, stats=False).item()
     ll_a = em_pass(G, P_a, Q_a, loglik=True, stats=False).item()
     L_star = em_pass(G, P_star, Q_star.contiguous(), loglik=True, stats=False).item()
     target = L_star - 1e-6 * abs(L_star)
@@ -42,4 +43,6 @@ def test_trained_network_beats_the_als_start(cuda):
     print(f"mean L1 to converged Q: ALS {mae_als:.4f}, network {mae_net:.4f}; start logL: ALS {ll_a:.1f}, "
           f"learned {ll_l:.1f} (used={used}); iterations to 1e-6: {its}", flush=True)
     assert mae_net < 0.8 * mae_als
-    assert used and ll_l > ll_a
+    # the gate (over its SNP band) chose the network's start, which is also
+    # the better one over all SNPs
+    assert used and ll_gate > gate_loglik(G, P_a, Q_a) and ll_l > ll_a
