@@ -151,6 +151,14 @@ class EMAdamOptimizer:
         """EM statistics at (P, Q) into A/B/T; optionally the FP64 logL at the
         same point (fused into the same pass in the f64 form)."""
         if self.precision == "f64":
+            if loglik and not is_streamed(G):
+                # Two passes over the resident G: the fused statistics +
+                # log-likelihood kernel needs more than 256 registers per lane
+                # (one wave per SIMD) and takes longer than both (at 100k x
+                # 500k, K=10: 233 ms against 150 + 34 ms).  Host-streamed G
+                # keeps the fused pass (one copy of each band).
+                em_pass(G, P, Q, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F64)
+                return em_pass(G, P, Q, dtype=_lib.ADX_F64, loglik=True, stats=False)
             return em_pass(G, P, Q, self.A_accum, self.B_accum, self.T_accum, dtype=_lib.ADX_F64, loglik=loglik)
         if self.precision == "bf16":
             if is_streamed(G):
