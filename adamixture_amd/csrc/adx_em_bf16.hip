@@ -390,13 +390,14 @@ struct Bf16Grid {
 };
 
 // K > 16: one launch per 16-wide k block at CW = 4 (H over the full K in each).
-// 8-wave workgroups for KH = 1 and, at CW = 4, KH = 2 (the KH = 2, CW = 8
-// kernel needs more than 256 registers per lane, one wave per SIMD).
+// K = 11..16 (KH = 2) also runs at CW = 4: at CW = 8 that kernel needs more
+// than 256 registers per lane (one wave per SIMD) and took 81.9 ms against
+// 67.7 ms at N=200k, M=600k, K=12.  8-wave workgroups for K <= 16.
 static Bf16Grid bf16_grid(int64_t M, int64_t N, int K) {
     const EmTiles tl = em_tiles(ADX_BF16);
     Bf16Grid gd;
-    gd.cw = K > 16 ? 4 : tl.cw;
-    gd.waves = (K > 16 || (bf16_kh(K) == 2 && gd.cw == 8)) ? 4 : tl.waves;
+    gd.cw = bf16_kh(K) >= 2 ? 4 : tl.cw;
+    gd.waves = K > 16 ? 4 : tl.waves;
     gd.nbx = ceil_div(M, 16 * gd.cw * gd.waves);
     int64_t s = ceil_div(tl.target_wgs, gd.nbx);
     const int64_t max_s = ceil_div(N, 64);
@@ -450,7 +451,10 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
                     ADX_BF16_LAUNCH(4, 8);
                     done = true;
                 }
-            } else if (gd.cw == 8) {
+            }
+        }
+        if constexpr (KH == 1) {
+            if (!done && gd.cw == 8) {
                 ADX_BF16_LAUNCH(8, 4);
                 done = true;
             }

@@ -95,9 +95,9 @@ int adx_synth_packed(const double *P, const double *Q, int64_t M, int64_t N, int
  * ADX_F64), waves = wavefronts per workgroup (4 or 8: a workgroup owns
  * 16 cw waves SNPs, and the partial-T slices shrink as that grows) and
  * target_wgs = workgroups the sample range is split to reach (64..65536).
- * waves applies to the statistics pass with K <= 16 (and, in the BF16 form,
- * not to K > 10 at cw = 8); passes that carry the log-likelihood run 4-wave
- * workgroups.  Process-wide and not synchronised between host threads
+ * waves applies to the statistics pass with K <= 16; passes that carry the
+ * log-likelihood run 4-wave workgroups.  cw is 4 for K > 16, and in the
+ * BF16 form for K > 10 (two BF16 k slices).  Process-wide and not synchronised between host threads
  * (set it before passes run on other threads); affects later launches and
  * adx_em_workspace_bytes (query the workspace after changing it).  Defaults:
  * F32 (8, 4, 8192), F64 (4, 4, 4096), BF16 (8, 4, 8192). */
