@@ -84,23 +84,64 @@ def initialize_k(G, N: int, M: int, K: int, f, seed: int, power: int, tol_svd: f
     return P, Q, logl
 
 
+# SVD settings of a rank-2K SVD the learned start runs when the K_max SVD of
+# setup has fewer than 2K components: the reference's defaults
+# (adamixture/entry.py:65-66).
+_WARM_SVD_POWER, _WARM_SVD_TOL = 5, 0.1
+_WARM_MODELS: dict = {}
+
+
+def _learned_start(G, N: int, M: int, K: int, U_max, S_max, V_max, f, seed: int, max_als: int, tol_als: float,
+                   dev, chunk_size: int, threads_per_block: int):
+    """(P, Q) of the learned warm start (learned_init.learned_warm_start,
+    with its fallback to SVD + ALS), or None when the shipped network does
+    not cover K or its weights are absent.  Features need a rank-2K SVD: the
+    leading 2K components of setup's K_max SVD when it has them, else a
+    rank-2K SVD of its own."""
+    from .learned_init import K_MAX, WEIGHTS_PATH, WarmStartMLP, learned_warm_start
+    if K > K_MAX or not WEIGHTS_PATH.exists():
+        log.warning(f"    No learned warm start for K={K} (the network covers K <= {K_MAX}); using SVD + ALS.")
+        return None
+    model = _WARM_MODELS.get(dev)
+    if model is None:
+        model = _WARM_MODELS[dev] = WarmStartMLP.load(dev)
+    if U_max is not None and U_max.shape[1] >= 2 * K:
+        U2, S2, V2 = U_max[:, : 2 * K].contiguous(), S_max[: 2 * K].contiguous(), V_max[:, : 2 * K].contiguous()
+    else:
+        U2, S2, V2 = SVD_gpu(G, N, M, f, 2 * K, seed, _WARM_SVD_POWER, _WARM_SVD_TOL, chunk_size, dev,
+                             threads_per_block)
+    P, Q, _, used = learned_warm_start(as_packed(G, M, dev), U2, S2, V2, f, K, model, seed, max_als=max_als,
+                                       tol_als=tol_als)
+    log.info(f"    Warm start for K={K}: {'learned' if used else 'SVD + ALS (fallback)'}.")
+    return P.contiguous(), Q.contiguous()
+
+
 def train_k(G, N: int, M: int, K: int, U_max, S_max, V_max, f, seed: int, lr: float, beta1: float,
             beta2: float, reg_adam: float, max_iter: int, check: int, max_als: int, tol_als: float,
             lr_decay: float, min_lr: float, chunk_size: int, patience: int, tol_adam: float, device_obj,
             threads_per_block: int, original: bool = False, rtol: float = 1e-7, Q_hist: int = 3,
             init_original: str = "em", em_init_steps: int = 5, P_init=None, Q_init=None,
-            precision: str | None = None):
+            precision: str | None = None, warm_start: str | None = None):
     """adamixture.py:253-352 (GPU branches; same parameters, order and
     defaults) -> (P, Q) device tensors.  The rank-K initialisation slices the
     K_max SVD of ``setup`` (U_max[:, :K], S_max[:K], V_max[:, :K]) as the
-    reference does; ``precision`` (project extension) selects the Adam-EM
-    pass form."""
+    reference does.  Project extensions: ``precision`` selects the Adam-EM
+    pass form; ``warm_start="learned"`` starts from the learned warm start
+    (with its SVD + ALS fallback) instead of the SVD + ALS start."""
     dev = _require_gpu(device_obj)
+    if warm_start not in (None, "svd_als", "learned"):
+        raise ValueError("warm_start must be None, 'svd_als' or 'learned'")
     if P_init is not None and Q_init is not None:
         P = torch.as_tensor(P_init, dtype=torch.float64, device=dev).contiguous()
         Q = torch.as_tensor(Q_init, dtype=torch.float64, device=dev).contiguous()
     else:
-        if original and init_original == "em":
+        start = None
+        if warm_start == "learned" and not (original and init_original == "em"):
+            start = _learned_start(G, N, M, K, U_max, S_max, V_max, f, seed, max_als, tol_als, dev, chunk_size,
+                                   threads_per_block)
+        if start is not None:
+            P, Q = start
+        elif original and init_original == "em":
             P, Q = initialize_em_gpu(G, seed, M, N, K, dev, chunk_size, threads_per_block, em_init_steps)
         else:
             U, S, V = U_max[:, :K].contiguous(), S_max[:K].contiguous(), V_max[:, :K].contiguous()

@@ -190,3 +190,38 @@ def test_driver_k_sweep_slices_the_k_max_svd(cuda, demo, fixtures):
     assert P4.shape == (M, 4) and np.isfinite(ll4)
     assert ll4 == pytest.approx(O.loglik_gpu(fixtures["reader_G_packed"], P4.cpu().numpy(), Q4.cpu().numpy(), M),
                                 rel=1e-12)
+
+
+def test_driver_learned_warm_start(cuda, demo, fixtures):
+    """train_k(..., warm_start="learned") (project extension): K = 3 takes the
+    leading 2K = 6 components of the K_max = 7 SVD, K = 7 runs its own rank-14
+    SVD; each equals learned_warm_start + the optimizer on the same inputs,
+    and converges to a model within the reference's tolerance of the plain
+    start's (logL within 50, adamixture tests/config.py:24-26)."""
+    from adamixture_amd.genotypes import as_packed
+    from adamixture_amd.learned_init import WarmStartMLP, learned_warm_start
+    from adamixture_amd.svd_gpu import SVD_gpu
+    Gt, N, M = demo
+    dev, tpb, f, U, S, V, G = setup(Gt, N, M, 7, SEED, POWER, TOL_SVD, CHUNK, "cuda:0")
+    lr, b1, b2, eps = ADAM
+    model = WarmStartMLP.load(dev)
+    Gp = fixtures["reader_G_packed"]
+    for K in (3, 7):
+        print(f"case: learned warm start K={K}", flush=True)
+        P, Q = _train(G, N, M, K, U, S, V, f, dev, tpb, warm_start="learned")
+        if 2 * K <= U.shape[1]:
+            U2, S2, V2 = U[:, : 2 * K].contiguous(), S[: 2 * K].contiguous(), V[:, : 2 * K].contiguous()
+        else:
+            U2, S2, V2 = SVD_gpu(G, N, M, f, 2 * K, SEED, POWER, TOL_SVD, CHUNK, dev, tpb)
+        P0, Q0, _, used = learned_warm_start(as_packed(G, M, dev), U2, S2, V2, f, K, model, SEED)
+        P2, Q2 = optimize_parameters_gpu(G, P0.contiguous(), Q0.contiguous(), lr, b1, b2, eps, MAX_ITER, CHECK, K, M,
+                                         N, LR_DECAY, MIN_LR, PATIENCE, TOL, dev, CHUNK, tpb)
+        assert torch.equal(P, P2) and torch.equal(Q, Q2)
+        Pn, Qn = P.cpu().numpy(), Q.cpu().numpy()
+        assert np.isfinite(Pn).all() and Pn.min() >= 1e-5 - 1e-15 and Pn.max() <= 1 - 1e-5 + 1e-15
+        np.testing.assert_allclose(Qn.sum(1), 1.0, atol=1e-9)
+        Pp, Qp = _train(G, N, M, K, U, S, V, f, dev, tpb)
+        ll_w = O.loglik_gpu(Gp, Pn, Qn, M)
+        ll_p = O.loglik_gpu(Gp, Pp.cpu().numpy(), Qp.cpu().numpy(), M)
+        print(f"K={K}: learned start used {used}; converged logL {ll_w:.2f} vs plain {ll_p:.2f}", flush=True)
+        assert abs(ll_w - ll_p) <= 50.0
