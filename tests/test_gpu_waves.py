@@ -108,3 +108,69 @@ def test_8_wave_pass_is_deterministic(cuda, tiles, precision, cw):
         PE, QE = opt.run_em_step(G, P, Q, G.M)
         out.append((PE.clone(), QE.clone()))
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
+def _stats(G, P, Q, dtype, ws_bytes, mode):
+    """Statistics-only pass through the C ABI: resident with a workspace of
+    ws_bytes, host-streamed with the same workspace, or allocation-free."""
+    dev = torch.device("cuda:0")
+    M, K = P.shape
+    N = Q.shape[0]
+    tdt = torch.float64 if dtype == _lib.ADX_F64 else torch.float32
+    Pd, Qd = torch.from_numpy(P).to(dev, tdt), torch.from_numpy(Q).to(dev, tdt)
+    A, B, T = (torch.zeros(r, K, dtype=tdt, device=dev) for r in (M, M, N))
+    args = (M, N, K, Pd.data_ptr(), K, Qd.data_ptr(), K, A.data_ptr(), B.data_ptr(), T.data_ptr(),
+            _lib.ADX_EM_STATS, None)
+    st = _lib.stream_handle(dev)
+    if mode == "noalloc":
+        _lib.call("adx_em_pass_noalloc", dtype, G.data.data_ptr(), G.ldg, *args, st)
+    else:
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        if mode == "streamed":
+            band = _lib.load().adx_em_band_snps(dtype, M, N, K, _lib.ADX_EM_STATS, ws_bytes, 0)
+            nst = 2 * (-(-((band + 3) // 4) * G.ldg // 256) * 256)
+            stg = torch.empty(nst, dtype=torch.uint8, device=dev)
+            cs = torch.cuda.Stream(device=dev)
+            _lib.call("adx_em_pass_host", dtype, G.host.data_ptr(), G.ldg, *args, ws.data_ptr(), ws_bytes,
+                      stg.data_ptr(), nst, st, cs.cuda_stream)
+        else:
+            _lib.call("adx_em_pass", dtype, G.data.data_ptr(), G.ldg, *args, ws.data_ptr(), ws_bytes, st)
+    torch.cuda.synchronize(dev)
+    return A, B, T
+
+
+@pytest.mark.parametrize("dtype,cw", [(_lib.ADX_F64, 4), (_lib.ADX_F32, 8)])
+@pytest.mark.parametrize("frac", [1, 4])
+def test_8_wave_streamed_pass_equals_resident(cuda, tiles, dtype, cw, frac):
+    """The SNP-band plan and the host-streamed pass at 8 waves: with the same
+    workspace (one band, or a quarter of it: several bands) the streamed
+    statistics are bit-identical to the resident ones, and match the oracle."""
+    tiles("f64" if dtype == _lib.ADX_F64 else "f32", cw, 8, 4096)
+    M, N, K = 12003, 1501, 9
+    print(f"case: 8-wave streamed dtype={dtype} frac={frac}", flush=True)
+    P, Q, Gd = _case(M, N, K, 0.02, 21, cuda)
+    Gh = GT.HostPackedGenotypes.from_packed(Gd.to_host_packed(), M, cuda)
+    ws = _lib.load().adx_em_workspace_bytes(dtype, M, N, K) // frac
+    res = _stats(Gd, P, Q, dtype, ws, "resident")
+    strm = _stats(Gh, P, Q, dtype, ws, "streamed")
+    assert all(torch.equal(a, b) for a, b in zip(res, strm))
+    tdt = np.float64 if dtype == _lib.ADX_F64 else np.float32
+    Pc, Qc = P.astype(tdt).astype(np.float64), Q.astype(tdt).astype(np.float64)
+    tol = 1e-12 if dtype == _lib.ADX_F64 else 2e-4
+    for got, want in zip(res, O.em_stats_gpu(Gd.to_host_packed(), Pc, Qc, M)[:3]):
+        np.testing.assert_allclose(got.double().cpu().numpy(), want, rtol=tol, atol=tol * np.abs(want).max())
+
+
+def test_8_wave_noalloc_pass_equals_workspace_pass(cuda, tiles):
+    """adx_em_pass_noalloc plans its bands for the 8-wave shape too: equal to
+    adx_em_pass with a workspace of the fixed scratch's size, twice in a row
+    (the second call on a smaller shape)."""
+    tiles("f32", 8, 8, 4096)
+    for M, N, K in ((20011, 2001, 7), (3001, 333, 5)):
+        print(f"case: 8-wave noalloc M={M} N={N} K={K}", flush=True)
+        P, Q, Gd = _case(M, N, K, 0.01, 22, cuda)
+        ref = _stats(Gd, P, Q, _lib.ADX_F32, int(_lib.load().adx_em_workspace_bytes(_lib.ADX_F32, M, N, K)),
+                     "resident")
+        got = _stats(Gd, P, Q, _lib.ADX_F32, 0, "noalloc")
+        for a, b in zip(got, ref):
+            np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=1e-5, atol=1e-5 * float(b.abs().max()))
