@@ -133,17 +133,39 @@ __device__ inline void project_simplex_row(const double* x, double* out, double*
 // takes a single log at the end.  The result differs from the sum of FP64
 // logs only by the product's roundings (<= 1.1e-16 each, random-walk
 // accumulated), which is below the FP64 summation error of that sum.
+//
+// The factor is formed without selects: with s1 = [code != 2] and
+// s2 = [code is 0 or 3] it is |H' - s1| |H' - s2|, where |H - 1| is exactly
+// the rounded 1 - H (round to nearest is symmetric) and |H - 0| = H, and
+// H' is H with its high word cleared for a missing code: a subnormal, so
+// that both differences are exactly -1 and the factor exactly 1.  Every
+// factor equals the select form's f1 * f2 (or 1) bit for bit.
+struct CodeMasks {
+    // bit 2i of each word describes the 2-bit code in field i of a genotype word
+    uint32_t obs;  // code != 3 (observed)
+    uint32_t s1;   // code != 2
+    uint32_t s2;   // code == 0 or code == 3
+};
+__device__ __forceinline__ CodeMasks code_masks(uint32_t w) {
+    const uint32_t hi = w >> 1;
+    return CodeMasks{~(w & hi), ~(hi & ~w), ~(w ^ hi)};
+}
+
 struct LogProd {
     double m = 1.0;
     int64_t e = 0;
-    // CLAMP = false: the caller knows the clamp is a no-op (H in range).
+    // The factor of the genotype whose code sits at bits p, p + 1 of the
+    // word cm was made from.  CLAMP = false: the caller knows the clamp is a
+    // no-op (H in range).
     template <bool CLAMP = true>
-    __device__ __forceinline__ void mul(double h, uint32_t code) {
+    __device__ __forceinline__ void mul(double h, const CodeMasks& cm, int p) {
         const double hc = CLAMP ? fmin(fmax(h, 1e-10), 1.0 - 1e-10) : h;
-        const double u = 1.0 - hc;
-        const double f1 = code == 2u ? hc : u;
-        const double f2 = code == 0u ? u : hc;
-        m *= code == 3u ? 1.0 : f1 * f2;
+        const uint32_t keep = (uint32_t)((int32_t)(cm.obs << (31 - p)) >> 31);
+        const double hk = __longlong_as_double(
+            (long long)((uint64_t)__double_as_longlong(hc) & (((uint64_t)keep << 32) | 0xFFFFFFFFull)));
+        const double s1 = (double)((cm.s1 >> p) & 1u);
+        const double s2 = (double)((cm.s2 >> p) & 1u);
+        m *= fabs(hk - s1) * fabs(hk - s2);
     }
     // a factor of another product of the same form (>= 1e-20)
     __device__ __forceinline__ void mul_factor(double f) { m *= f; }

@@ -83,8 +83,8 @@ constexpr int TSX = 17;
 
 // The pass body.  LLFAST: the log-likelihood factors skip the clamp of H to
 // [1e-10, 1 - 1e-10], which ll_domain_kernel has shown to be a no-op for
-// this (P, Q) (LogProd::mul), saving three of the ~16 FP64 VALU
-// instructions per genotype.  The LDS arrays are the kernel's.
+// this (P, Q) (LogProd::mul), saving three of its FP64 VALU instructions
+// per genotype.  The LDS arrays are the kernel's.
 template <typename T, int KS, int CW, int WV, bool STATS, bool LL, bool LLFAST>
 __device__ __forceinline__ void em_pass_body(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const T* __restrict__ P,
@@ -179,6 +179,11 @@ __device__ __forceinline__ void em_pass_body(
             }
             acc_t accT = acc_t{0, 0, 0, 0};
             T sb[4] = {T(0), T(0), T(0), T(0)};
+            CodeMasks cm[CW / 4];
+            if (LL) {
+#pragma unroll
+                for (int t = 0; t < CW / 4; ++t) cm[t] = code_masks(gw[t][rw]);
+            }
 
 #pragma unroll
             for (int cb = 0; cb < CW; ++cb) {
@@ -192,7 +197,7 @@ __device__ __forceinline__ void em_pass_body(
                 for (int j = 0; j < 4; ++j) {
                     const uint32_t code = (w >> (8 * j)) & 3u;
                     if (STATS) em_terms<T>(h[j], code, a[j], b[j]);
-                    if (LL) lp.mul<!LLFAST>((double)h[j], code);
+                    if (LL) lp.mul<!LLFAST>((double)h[j], cm[t], 2 * f + 8 * j);
                 }
                 // 8 factors (two column blocks) between renormalisations: the
                 // mantissa stays >= 1e-160; the rescale is exact, so the result

@@ -2,8 +2,9 @@
 manager's placement decision (device, host-streamed, or out of memory) with
 a faked free-memory value (the reference's
 own approach to testing it, adamixture/src/utils.py:311-429), the OOM
-classifier and chunk halving (utils.py:19-89), and the .P / .Q writer
-format (utils.py:252-273)."""
+classifier and chunk halving (utils.py:19-89), the .P / .Q writer
+format (utils.py:252-273), and the IEEE identity behind the log-likelihood
+pass's select-free genotype factor."""
 import numpy as np
 import pytest
 import torch
@@ -178,3 +179,43 @@ def test_packed_cache_is_keyed_by_tensor_identity():
     del a
     gc.collect()
     assert len(GT._CACHE) == n - 1
+
+
+def _code_masks(w):
+    """Bitwise word masks of the log-likelihood pass: bit 2i of each is a flag of
+    the 2-bit code in field i (observed; code != 2; code is 0 or 3)."""
+    hi = w >> np.uint32(1)
+    return ~(w & hi), ~(hi & ~w), ~(w ^ hi)
+
+
+def test_select_free_loglik_factor_equals_the_select_form():
+    """The log-likelihood pass forms each genotype's factor without selects:
+    |H' - s1| |H' - s2| with s1 = [code != 2], s2 = [code in {0, 3}] read from
+    word masks, and H' = H with its high word cleared for a missing code.  In
+    IEEE FP64 (round to nearest) that equals the select form (1-H)^2, H (1-H),
+    H^2 and 1 bit for bit, because |H - 1| is the rounded 1 - H and a
+    subnormal H' gives H' - 1 = -1 exactly.  Checked on every code of random
+    genotype words, with H at the domain's edges, at 1/2 and around it."""
+    rng = np.random.default_rng(3)
+    n = 200_000
+    h = np.concatenate([rng.random(n - 8), [1e-10, 1 - 1e-10, 2e-10, 1 - 2e-10, 0.5,
+                                            np.nextafter(0.5, 0), np.nextafter(0.5, 1), 0.25]])
+    words = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    obs, m1, m2 = _code_masks(words)
+    one = np.uint32(1)
+    for p in range(0, 32, 2):
+        code = (words >> np.uint32(p)) & np.uint32(3)
+        u = 1.0 - h
+        want = np.where(code == 3, 1.0, np.where(code == 2, h, u) * np.where(code == 0, u, h))
+        keep = ((obs >> np.uint32(p)) & one).astype(np.uint64) * np.uint64(0xFFFFFFFF)
+        hk = (h.view(np.uint64) & ((keep << np.uint64(32)) | np.uint64(0xFFFFFFFF))).view(np.float64)
+        s1 = ((m1 >> np.uint32(p)) & one).astype(np.float64)
+        s2 = ((m2 >> np.uint32(p)) & one).astype(np.float64)
+        got = np.abs(hk - s1) * np.abs(hk - s2)
+        np.testing.assert_array_equal(got.view(np.uint64), want.view(np.uint64))
+    # a planted mistake: the masks of the wrong field give other factors
+    code = words & np.uint32(3)
+    s1_wrong = ((m1 >> np.uint32(2)) & one).astype(np.float64)
+    wrong = np.abs(h - s1_wrong) * np.abs(h - ((m2 & one).astype(np.float64)))
+    right = np.where(code == 2, h, 1.0 - h) * np.where(code == 0, 1.0 - h, h)
+    assert np.mean((wrong != right)[code != 3]) > 0.3
