@@ -39,19 +39,32 @@
 
 namespace adx {
 
-template <typename T>
-__device__ __forceinline__ void em_terms(T h, uint32_t code, T& a, T& b);
+// The weights g and 2 - g of a genotype (0 for a missing code) are read
+// from two masked copies of its genotype word, built once per word:
+// field i of ga holds g (0 for code 3), field i of gb holds 2 - g (0 for
+// code 3).  Same values as converting the code and selecting 0 for a
+// missing one, without a compare or a select per genotype.
+struct WeightWords {
+    uint32_t ga, gb;
+};
+__device__ __forceinline__ WeightWords weight_words(uint32_t w) {
+    const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u;
+    const uint32_t miss = lo & hi;
+    return WeightWords{w & ~(miss | (miss << 1)), ((~(lo | hi) & 0x55555555u) << 1) | (lo & ~hi)};
+}
 
-// FP32: one reciprocal of h(1-h) gives both 1/h and 1/(1-h).
+template <typename T>
+__device__ __forceinline__ void em_terms(T h, T ga, T gb, T& a, T& b);
+
+// FP32: one reciprocal of h(1-h) gives both 1/h and 1/(1-h).  The clamp is
+// one med3 (equal to fmin(fmax(h, lo), hi) for every non-NaN h).
 template <>
-__device__ __forceinline__ void em_terms<float>(float h, uint32_t code, float& a, float& b) {
-    const float hc = fminf(fmaxf(h, 1e-5f), 1.0f - 1e-5f);
+__device__ __forceinline__ void em_terms<float>(float h, float ga, float gb, float& a, float& b) {
+    const float hc = __builtin_amdgcn_fmed3f(h, 1e-5f, 1.0f - 1e-5f);
     const float u = 1.0f - hc;
     const float r = __builtin_amdgcn_rcpf(hc * u);
-    const float ca = code == 3u ? 0.0f : (float)code;
-    const float cb = code == 3u ? 0.0f : 2.0f - (float)code;
-    a = ca * (u * r);
-    b = cb * (hc * r);
+    a = ga * (u * r);
+    b = gb * (hc * r);
 }
 
 // FP64: the reference's quotients g / H and (2-g) / (1-H)
@@ -69,13 +82,12 @@ __device__ __forceinline__ double recip_nr(double x) {
 }
 
 template <>
-__device__ __forceinline__ void em_terms<double>(double h, uint32_t code, double& a, double& b) {
+__device__ __forceinline__ void em_terms<double>(double h, double ga, double gb, double& a, double& b) {
     const double hc = fmin(fmax(h, 1e-5), 1.0 - 1e-5);
     const double u = 1.0 - hc;
     const double r = recip_nr(hc * u);
-    const double g = (double)code;
-    a = code == 3u ? 0.0 : g * (u * r);
-    b = code == 3u ? 0.0 : (2.0 - g) * (hc * r);
+    a = ga * (u * r);
+    b = gb * (hc * r);
 }
 
 // Row stride of the per-wave transpose tile (see em_pass_body).
@@ -180,9 +192,11 @@ __device__ __forceinline__ void em_pass_body(
             acc_t accT = acc_t{0, 0, 0, 0};
             T sb[4] = {T(0), T(0), T(0), T(0)};
             CodeMasks cm[CW / 4];
-            if (LL) {
+            WeightWords ww[CW / 4];
 #pragma unroll
-                for (int t = 0; t < CW / 4; ++t) cm[t] = code_masks(gw[t][rw]);
+            for (int t = 0; t < CW / 4; ++t) {
+                if (LL) cm[t] = code_masks(gw[t][rw]);
+                if (STATS) ww[t] = weight_words(gw[t][rw]);
             }
 
 #pragma unroll
@@ -191,12 +205,13 @@ __device__ __forceinline__ void em_pass_body(
                 acc_t h = acc_t{0, 0, 0, 0};
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) h = MM::run(qH[ks], pH[cb][ks], h);
-                const uint32_t w = gw[t][rw] >> (2 * f);
+                // bytes j of wa, wb: g and 2 - g of sample j's genotype
+                const uint32_t wa = (ww[t].ga >> (2 * f)) & 0x03030303u;
+                const uint32_t wb = (ww[t].gb >> (2 * f)) & 0x03030303u;
                 T a[4], b[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const uint32_t code = (w >> (8 * j)) & 3u;
-                    if (STATS) em_terms<T>(h[j], code, a[j], b[j]);
+                    if (STATS) em_terms<T>(h[j], (T)((wa >> (8 * j)) & 0xFFu), (T)((wb >> (8 * j)) & 0xFFu), a[j], b[j]);
                     if (LL) lp.mul<!LLFAST>((double)h[j], cm[t], 2 * f + 8 * j);
                 }
                 // 8 factors (two column blocks) between renormalisations: the

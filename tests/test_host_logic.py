@@ -3,8 +3,8 @@ manager's placement decision (device, host-streamed, or out of memory) with
 a faked free-memory value (the reference's
 own approach to testing it, adamixture/src/utils.py:311-429), the OOM
 classifier and chunk halving (utils.py:19-89), the .P / .Q writer
-format (utils.py:252-273), and the IEEE identity behind the log-likelihood
-pass's select-free genotype factor."""
+format (utils.py:252-273), and the bit identities behind the EM passes'
+select-free genotype weights and log-likelihood factors."""
 import numpy as np
 import pytest
 import torch
@@ -219,3 +219,25 @@ def test_select_free_loglik_factor_equals_the_select_form():
     wrong = np.abs(h - s1_wrong) * np.abs(h - ((m2 & one).astype(np.float64)))
     right = np.where(code == 2, h, 1.0 - h) * np.where(code == 0, 1.0 - h, h)
     assert np.mean((wrong != right)[code != 3]) > 0.3
+
+
+def test_weight_words_give_g_and_two_minus_g():
+    """The statistics passes read the EM weights g and 2 - g (0 for a missing
+    code; em_adam_gpu.py:57-67 masks them) from two masked copies of each
+    genotype word instead of converting the code and selecting: every 2-bit
+    field of the first holds g, of the second 2 - g, on random words and on
+    the four all-equal words."""
+    rng = np.random.default_rng(4)
+    words = np.concatenate([rng.integers(0, 2**32, size=100_000, dtype=np.uint64).astype(np.uint32),
+                            np.array([0, 0x55555555, 0xAAAAAAAA, 0xFFFFFFFF], dtype=np.uint32)])
+    m55 = np.uint32(0x55555555)
+    lo, hi = words & m55, (words >> np.uint32(1)) & m55
+    miss = lo & hi
+    ga = words & ~(miss | (miss << np.uint32(1)))
+    gb = ((~(lo | hi) & m55) << np.uint32(1)) | (lo & ~hi)
+    for p in range(0, 32, 2):
+        code = (words >> np.uint32(p)) & np.uint32(3)
+        g_want = np.where(code == 3, 0, code)
+        b_want = np.where(code == 3, 0, 2 - code.astype(np.int64))
+        np.testing.assert_array_equal((ga >> np.uint32(p)) & np.uint32(3), g_want)
+        np.testing.assert_array_equal((gb >> np.uint32(p)) & np.uint32(3), b_want)
