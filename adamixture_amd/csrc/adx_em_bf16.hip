@@ -10,9 +10,10 @@
 //          P_hi] (three-term split: |H - H_exact| <~ 3 * 2^-16 |H|); one
 //          MFMA for K <= 10, two for K <= 16.  The factor 8 (exact) gives
 //          h8 = 8 H.
-//   A^T += (Qt_hi + Qt_lo) . a'^T,  B^T += (Qt_hi + Qt_lo) . b'^T   (FP16)
-//          k = 32 samples (two 16-sample row blocks); Qt = 256 Q split into
-//          two FP16 terms, a' = a / 8 and b' = b / 8 rounded to FP16.
+//   A^T += Qt . a'^T,  B^T += Qt . b'^T   (FP16)
+//          k = 32 samples (two 16-sample row blocks); Qt = 256 Q rounded to
+//          FP16, a' = a / 8 and b' = b / 8 rounded to FP16 (the operands of
+//          each product carry the same 11-bit rounding, as P does in T).
 //   T   += [a' b'] . [P ; 1 - P]   (FP16)  k = 16 SNPs x {a', b'}: a' and b'
 //          of one genotype share one word (f16 pair), the 16 x 16 word tile
 //          is transposed through LDS (one 32-bit write per genotype, one
@@ -74,7 +75,7 @@ template <int KH, int CW, int WV>
 __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     const uint8_t* __restrict__ G, int64_t ldg, int64_t M, int64_t N, int K, const __bf16* __restrict__ Px,
     const __bf16* __restrict__ Qx, const _Float16* __restrict__ Qt, int64_t ldqt, int64_t n_per_split,
-    float* __restrict__ Tpart, float* __restrict__ ABpart, int kb0, int kbt) {
+    float* __restrict__ Tpart, float* __restrict__ ABpart, int kb0) {
     static_assert(CW % 4 == 0, "CW must be a multiple of 4");
     constexpr int LX = 32 * KH;  // row length of Px / Qx
     constexpr int TS = 20;       // row stride (words) of the per-wave transpose tile
@@ -144,20 +145,20 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     }
     // The Q operands of a 64-sample sub-tile are staged once per workgroup by
     // its first 256 threads (waves 0-3; the others skip the staging):
-    // thread t copies Qt chunk (row t >> 3, samples 8 (t & 7) .. + 7) and KH
-    // Qx chunks (16 bytes each) into LDS in lane-fragment order,
-    //   s_qt[(2 tp + hl) 64 + lane]          = Qt row (hl ? 16 + cl : cl), samples 16g + 8tp .. + 7
+    // thread t < 128 copies Qt chunk (row t >> 3, samples 8 (t & 7) .. + 7)
+    // and every thread KH Qx chunks (16 bytes each) into LDS in lane-fragment
+    // order,
+    //   s_qt[tp 64 + lane]                   = Qt row cl, samples 16g + 8tp .. + 7
     //   s_qx[((2 tp + rr) KH + kh) 64 + lane] = Qx row 16 (cl >> 2) + 4 (2tp + rr) + (cl & 3),
     //                                          columns 32 kh + 8 g .. + 7 (zero past N)
     // so each wave's 128-bit operand reads hit consecutive addresses.
-    __shared__ u32x4 s_qt[4 * 64];
+    __shared__ u32x4 s_qt[2 * 64];
     __shared__ u32x4 s_qx[4 * KH * 64];
     const int tid = threadIdx.x;
-    // Qt rows of this k block: hi rows kb0 .. kb0 + 15, lo rows 16 kbt + kb0 ..
-    const int qt_r = tid >> 3;
-    const _Float16* qt_src =
-        Qt + (int64_t)(qt_r < 16 ? kb0 + qt_r : 16 * kbt + kb0 + qt_r - 16) * ldqt + 8 * (tid & 7);
-    const int qt_dst = (2 * (tid & 1) + (tid >> 7)) * 64 + 16 * ((tid & 7) >> 1) + ((tid >> 3) & 15);
+    // Qt rows of this k block: kb0 .. kb0 + 15
+    const bool qt_stager = tid < 128;
+    const _Float16* qt_src = Qt + (int64_t)(kb0 + ((tid >> 3) & 15)) * ldqt + 8 * (tid & 7);
+    const int qt_dst = (tid & 1) * 64 + 16 * ((tid & 7) >> 1) + ((tid >> 3) & 15);
     int qx_row[KH], qx_col[KH], qx_dst[KH];
 #pragma unroll
     for (int j = 0; j < KH; ++j) {
@@ -173,7 +174,7 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     uint32_t gn[CW / 4][4];
     auto load_next = [&](int64_t s) {
         if (stager) {
-            qn_t = *reinterpret_cast<const u32x4*>(qt_src + s);
+            if (qt_stager) qn_t = *reinterpret_cast<const u32x4*>(qt_src + s);
 #pragma unroll
             for (int j = 0; j < KH; ++j) {
                 const int64_t r = s + qx_row[j];
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     };
     auto stage_next = [&](uint32_t (&gw)[CW / 4][4]) {
         if (stager) {
-            s_qt[qt_dst] = qn_t;
+            if (qt_stager) s_qt[qt_dst] = qn_t;
 #pragma unroll
             for (int j = 0; j < KH; ++j) s_qx[qx_dst[j]] = qn_x[j];
         }
@@ -217,8 +218,7 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
         // (g, cl) the 8 consecutive samples s0 + 16g + 8tp + i, i = 0..7.
 #pragma unroll
         for (int tp = 0; tp < 2; ++tp) {
-            const f16x8 qThi = __builtin_bit_cast(f16x8, s_qt[(2 * tp) * 64 + lane]);
-            const f16x8 qTlo = __builtin_bit_cast(f16x8, s_qt[(2 * tp + 1) * 64 + lane]);
+            const f16x8 qT = __builtin_bit_cast(f16x8, s_qt[tp * 64 + lane]);
             bf16x8 qH[2][KH];
 #pragma unroll
             for (int rr = 0; rr < 2; ++rr)
@@ -279,10 +279,8 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
                 // B operands of A^T, B^T: k = sample 8g + i (rr = i / 4, j = i % 4).
                 const f16x8 aB = __builtin_bit_cast(f16x8, u32x4{wa[0][0], wa[0][1], wa[1][0], wa[1][1]});
                 const f16x8 bB = __builtin_bit_cast(f16x8, u32x4{wb[0][0], wb[0][1], wb[1][0], wb[1][1]});
-                accA[cb] = mma_f16(qThi, aB, accA[cb]);
-                accA[cb] = mma_f16(qTlo, aB, accA[cb]);
-                accB[cb] = mma_f16(qThi, bB, accB[cb]);
-                accB[cb] = mma_f16(qTlo, bB, accB[cb]);
+                accA[cb] = mma_f16(qT, aB, accA[cb]);
+                accB[cb] = mma_f16(qT, bB, accB[cb]);
                 if (cb + 1 < CW) {
                     hc[0] = hn[0];
                     hc[1] = hn[1];
@@ -336,10 +334,10 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
 // Operand arrays of the pass from the FP64 master P (M x K) and Q (N x K):
 //   Px[m]  = [P_hi(0..K), P_lo, P_hi, 0 ...]    (32*KH bf16)
 //   Qx[i]  = 8 [Q_hi, Q_hi, Q_lo, 0 ...]        (32*KH bf16)
-//   Qt     = [hi rows 0..16KB-1 | lo rows 0..16KB-1] x ldqt, Qt[kp][i] = 256 Q[i][kp]
-//            split into two FP16 terms (KB = ceil(K / 16) k blocks)
-// with x_hi = bf16(x), x_lo = bf16(x - x_hi) (FP16 for Qt); zero outside the
-// valid range.  The power-of-two factors are exact.
+//   Qt     = rows 0..16KB-1 x ldqt, Qt[kp][i] = 256 Q[i][kp] rounded to FP16
+//            (KB = ceil(K / 16) k blocks)
+// with x_hi = bf16(x), x_lo = bf16(x - x_hi); zero outside the valid range.
+// The power-of-two factors are exact.
 __global__ void prep_px_kernel(const double* __restrict__ P, int64_t M, int K, int LX, __bf16* __restrict__ Px) {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
@@ -361,7 +359,7 @@ __global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, i
     if (i >= ldqt) return;
     __bf16* row = i < N ? Qx + i * LX : nullptr;
     for (int k = 0; k < 16 * KB; ++k) {
-        _Float16 thi = (_Float16)0.0f, tlo = (_Float16)0.0f;
+        _Float16 t = (_Float16)0.0f;
         if (i < N && k < K) {
             const float x = (float)Q[i * K + k];
             const __bf16 hi = (__bf16)(8.0f * x);
@@ -369,12 +367,9 @@ __global__ void prep_qx_kernel(const double* __restrict__ Q, int64_t N, int K, i
             row[k] = hi;
             row[K + k] = hi;
             row[2 * K + k] = lo;
-            const float y = 256.0f * x;
-            thi = (_Float16)y;
-            tlo = (_Float16)(y - (float)thi);
+            t = (_Float16)(256.0f * x);
         }
-        Qt[(int64_t)k * ldqt + i] = thi;
-        Qt[(int64_t)(16 * KB + k) * ldqt + i] = tlo;
+        Qt[(int64_t)k * ldqt + i] = t;
     }
     if (row)
         for (int e = 3 * K; e < LX; ++e) row[e] = (__bf16)0.0f;
@@ -436,7 +431,7 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
     const int kbt = bf16_kb(K);
 #define ADX_BF16_LAUNCH(cw, wv)                                                                               \
     em_pass_bf16_kernel<KH, cw, wv><<<grid, 64 * wv, 0, st>>>(G, ldg, M, N, K, Px, Qx, Qt, ldqt, gd.n_per_split, \
-                                                              Tpart, ABpart, 16 * kb, kbt)
+                                                              Tpart, ABpart, 16 * kb)
     for (int kb = 0; kb < kbt; ++kb) {
         bool done = false;
         if constexpr (KH <= 2) {
@@ -474,7 +469,7 @@ static void launch_bf16(const Bf16Grid& gd, hipStream_t st, const uint8_t* G, in
 static int64_t bf16_base_bytes(int64_t M, int64_t N, int K) {
     const int kh = bf16_kh(K);
     const int64_t ld = ceil_div(N, 64) * 64;
-    return ceil_div((int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 32 * bf16_kb(K) * ld), 256) * 256;
+    return ceil_div((int64_t)sizeof(__bf16) * (M * 32 * kh + N * 32 * kh + 16 * bf16_kb(K) * ld), 256) * 256;
 }
 
 }  // namespace adx

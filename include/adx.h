@@ -187,8 +187,8 @@ int adx_em_pass_noalloc(int dtype, const uint8_t *G, int64_t ldg, int64_t M, int
  * dtype ADX_F32, but the contractions run on 16x16x32 MFMA with FP32
  * accumulation from operand arrays that adx_em_prep_bf16 builds from the
  * FP64 P and Q: H on BF16 MFMA from a three-term hi/lo split; the EM weights
- * a, b rounded to FP16 and the A, B, T contractions on FP16 MFMA (Q split in
- * two FP16 terms).  Stated accuracy against the FP64 pass: 1e-3 relative on
+ * a, b rounded to FP16 and the A, B, T contractions on FP16 MFMA (Q and P
+ * rounded to FP16).  Stated accuracy against the FP64 pass: 1e-3 relative on
  * A, B, T.
  * The operand buffer also holds the per-workgroup partial sums of T
  * (ceil(M / (16 cw waves)) x N x K FP32, cw and waves from adx_set_em_tiles) and the
