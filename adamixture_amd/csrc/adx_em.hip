@@ -211,7 +211,8 @@ __device__ __forceinline__ void em_pass_body(
                 T a[4], b[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    if (STATS) em_terms<T>(h[j], (T)((wa >> (8 * j)) & 0xFFu), (T)((wb >> (8 * j)) & 0xFFu), a[j], b[j]);
+                    if (STATS)
+                        em_terms<T>(h[j], (T)((wa >> (8 * j)) & 0xFFu), (T)((wb >> (8 * j)) & 0xFFu), a[j], b[j]);
                     if (LL) lp.mul<!LLFAST>((double)h[j], cm[t], 2 * f + 8 * j);
                 }
                 // 8 factors (two column blocks) between renormalisations: the
