@@ -14,7 +14,9 @@
  *     cython/snp_reader.pyx:76-120): Mb = ceil(M/4) byte rows of `ldg` bytes,
  *     byte [j, i] holds SNPs 4j..4j+3 of sample i at bits 2k..2k+1; codes
  *     0/1/2 = allele count, 3 = missing.  Entry points that stream whole
- *     rows need ldg % 16 == 0; bytes i >= N of a row are padding.
+ *     rows need ldg % 16 == 0; bytes i >= N of a row are padding and must
+ *     hold missing codes (0xFF, as adx_repack_padded and the readers write
+ *     them): the log-likelihood pass counts any other padding code.
  *   - Matrices are row-major: P is M x K (stride ldp), Q is N x K (stride ldq).
  *   - dtype tags: ADX_F32 = 0, ADX_F64 = 1, ADX_BF16 = 2.
  *
