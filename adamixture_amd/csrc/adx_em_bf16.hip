@@ -95,7 +95,13 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     // SNPs; each (sample, k) is written once per wave and row block, and the
     // WV slices are summed in a fixed order at the flush (deterministic).
     // Rows padded to 17 floats so the 64 lanes of a write hit distinct banks.
-    __shared__ float s_tsum[WV][64 * 17];
+    // NB = 2 (KH <= 2): this and the staged Q chunks below are double
+    // buffered, so one workgroup barrier per sub-tile separates the writes of
+    // one buffer from the reads of the other; with KH > 2 the Q chunks are
+    // too large for two workgroups per CU and one buffer with two barriers
+    // is kept.
+    constexpr int NB = KH <= 2 ? 2 : 1;
+    __shared__ float s_tsum[NB][WV][64 * 17];
 
     const bf16x8 zero8 = bf16x8{};
     bf16x8 pH[CW][KH];
@@ -152,8 +158,8 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     //   s_qx[((2 tp + rr) KH + kh) 64 + lane] = Qx row 16 (cl >> 2) + 4 (2tp + rr) + (cl & 3),
     //                                          columns 32 kh + 8 g .. + 7 (zero past N)
     // so each wave's 128-bit operand reads hit consecutive addresses.
-    __shared__ u32x4 s_qt[2 * 64];
-    __shared__ u32x4 s_qx[4 * KH * 64];
+    __shared__ u32x4 s_qt[NB][2 * 64];
+    __shared__ u32x4 s_qx[NB][4 * KH * 64];
     const int tid = threadIdx.x;
     // Qt rows of this k block: kb0 .. kb0 + 15
     const bool qt_stager = tid < 128;
@@ -191,11 +197,11 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
             gn[t][3] = v.w;
         }
     };
-    auto stage_next = [&](uint32_t (&gw)[CW / 4][4]) {
+    auto stage_next = [&](uint32_t (&gw)[CW / 4][4], int buf) {
         if (stager) {
-            if (qt_stager) s_qt[qt_dst] = qn_t;
+            if (qt_stager) s_qt[buf][qt_dst] = qn_t;
 #pragma unroll
-            for (int j = 0; j < KH; ++j) s_qx[qx_dst[j]] = qn_x[j];
+            for (int j = 0; j < KH; ++j) s_qx[buf][qx_dst[j]] = qn_x[j];
         }
 #pragma unroll
         for (int t = 0; t < CW / 4; ++t)
@@ -204,8 +210,9 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
     };
     uint32_t gw[CW / 4][4];
     load_next(n0);
-    stage_next(gw);
+    stage_next(gw, 0);
     __syncthreads();
+    int pb = 0;  // buffer of the current sub-tile
 
     for (int64_t s0 = n0; s0 < n1; s0 += 64) {
         // The next sub-tile's genotype words and Q chunks are in flight
@@ -218,13 +225,13 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
         // (g, cl) the 8 consecutive samples s0 + 16g + 8tp + i, i = 0..7.
 #pragma unroll
         for (int tp = 0; tp < 2; ++tp) {
-            const f16x8 qT = __builtin_bit_cast(f16x8, s_qt[tp * 64 + lane]);
+            const f16x8 qT = __builtin_bit_cast(f16x8, s_qt[pb][tp * 64 + lane]);
             bf16x8 qH[2][KH];
 #pragma unroll
             for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
                 for (int kh = 0; kh < KH; ++kh)
-                    qH[rr][kh] = __builtin_bit_cast(bf16x8, s_qx[((2 * tp + rr) * KH + kh) * 64 + lane]);
+                    qH[rr][kh] = __builtin_bit_cast(bf16x8, s_qx[pb][((2 * tp + rr) * KH + kh) * 64 + lane]);
             floatx4 accT[2] = {floatx4{0, 0, 0, 0}, floatx4{0, 0, 0, 0}};
 
             // H of column block cb + 1 is issued before the weights of block
@@ -290,12 +297,18 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
             for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    if (kb0 + cl < K) s_tsum[wave][(16 * g + 4 * (2 * tp + rr) + j) * 17 + cl] = accT[rr][j];
+                    if (kb0 + cl < K) s_tsum[pb][wave][(16 * g + 4 * (2 * tp + rr) + j) * 17 + cl] = accT[rr][j];
         }
 
-        __syncthreads();
-        // every wave is past its reads of the staged Q chunks
-        stage_next(gw);
+        if constexpr (NB == 2) {
+            // the other buffers were last read before the previous barrier
+            stage_next(gw, pb ^ 1);
+            __syncthreads();
+        } else {
+            __syncthreads();
+            // every wave is past its reads of the staged Q chunks
+            stage_next(gw, 0);
+        }
         // T of these 64 samples over this workgroup's SNPs: one plain store
         // into the workgroup's slice of the partial buffer (every (slice,
         // sample) is written exactly once), summed by tpart_reduce_kernel.
@@ -306,9 +319,12 @@ __global__ __launch_bounds__(64 * WV) void em_pass_bf16_kernel(
             const int64_t smp = s0 + (e >> 4);
             const int ex = (e >> 4) * 17 + kp;
             if (kb0 + kp < K && smp < N)
-                Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] = sum_wave_slices<WV>(s_tsum, ex);
+                Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] = sum_wave_slices<WV>(s_tsum[pb], ex);
         }
-        __syncthreads();
+        if constexpr (NB == 2)
+            pb ^= 1;
+        else
+            __syncthreads();
     }
 
 #pragma unroll
