@@ -144,6 +144,10 @@ __device__ __forceinline__ void em_pass_body(
         accB[cb] = acc_t{0, 0, 0, 0};
     }
     LogProd lp;
+    // s_tsum holds two sets of WV slices, used by alternate sub-tiles: the
+    // barrier after a sub-tile's slices are written also separates the
+    // previous flush's reads of the other set from the next writes to it.
+    int pb = 0;
     __syncthreads();
 
     for (int64_t s0 = n0; s0 < n1; s0 += 64) {
@@ -243,7 +247,7 @@ __device__ __forceinline__ void em_pass_body(
                     v += __shfl_xor(v, 2);
                     v += __shfl_xor(v, 4);
                     v += __shfl_xor(v, 8);
-                    if (kb0 + cl < K) s_tsum[wave][(16 * g + 4 * rw + j) * 17 + cl] = accT[j] + v;
+                    if (kb0 + cl < K) s_tsum[pb * WV + wave][(16 * g + 4 * rw + j) * 17 + cl] = accT[j] + v;
                 }
             }
         }
@@ -260,14 +264,14 @@ __device__ __forceinline__ void em_pass_body(
                 const int64_t smp = s0 + (e >> 4);
                 const int ex = (e >> 4) * 17 + kp;
                 if (kb0 + kp < K && smp < N) {
-                    const T v = sum_wave_slices<WV>(s_tsum, ex);
+                    const T v = sum_wave_slices<WV>(s_tsum + pb * WV, ex);
                     if (Tpart)
                         Tpart[((int64_t)blockIdx.x * N + smp) * K + kb0 + kp] = v;
                     else
                         atomicAdd(&Tacc[smp * K + kb0 + kp], v);
                 }
             }
-            __syncthreads();
+            pb ^= 1;
         }
     }
 
@@ -325,8 +329,9 @@ __global__ __launch_bounds__(64 * WV) void em_pass_kernel(
     // SNPs.  Each wave writes each (sample, k) once per row block, so the
     // slices need no atomics and are summed in a fixed order at the flush.
     // Rows padded to 17 elements: the 64 lanes of a write (4 lane groups x
-    // 16 columns, rows 16 apart) fall on distinct banks.
-    __shared__ T s_tsum[STATS ? WV : 1][64 * 17];
+    // 16 columns, rows 16 apart) fall on distinct banks.  Two sets of slices
+    // (alternate sub-tiles): one workgroup barrier per sub-tile.
+    __shared__ T s_tsum[STATS ? 2 * WV : 1][64 * 17];
     __shared__ double s_ll[WV][1];
     // ll_fast (written by ll_domain_kernel earlier on the stream, the same
     // value for every thread): one uniform choice of the body per launch.
