@@ -17,8 +17,8 @@
 // per problem and reduction split; the rounding of the subtractions is
 // relative to S_w2, the scale of the Hessian entries).  Each thread owns one
 // problem's accumulators in registers and streams the 2-bit genotypes once:
-//   Q side: thread = sample (K <= 8) or two waves per 64 samples, each with
-//           half the Hessian rows (K = 9..16); workgroup = 256 or 128
+//   Q side: thread = sample (K <= 10) or two waves per 64 samples, each with
+//           half the Hessian rows (K = 11..16); workgroup = 256 or 128
 //           consecutive samples x a SNP range; one coalesced byte per 4
 //           SNPs, P rows are wave-uniform.
 //   P side: thread = SNP, workgroup = 256 SNPs; 256-sample chunks of the
@@ -478,8 +478,13 @@ static void launch_gh(int side, hipStream_t st, const uint8_t* G, int64_t ldg, i
     const int64_t Mb = ceil_div(M, 4);
     if (side == 0) {
         // rows of the Hessian triangle owned by the first of two waves (0: one
-        // thread per sample); chosen so both roles do about equal work
-        constexpr int KS = KM <= 8 ? 0 : KM <= 10 ? 2 : KM <= 14 ? 3 : 4;
+        // thread per sample); chosen so both roles do about equal work.  One
+        // thread per sample up to K = 10 (218 VGPRs, two waves per SIMD): at
+        // N=100k, M=500k, K=10 it took 216 ms against 281 ms for the split,
+        // which forms H and the weights in both waves; at K = 12 it needs
+        // more than 256 VGPRs and the split is faster (902 against 965 ms
+        // at N=200k, M=600k).
+        constexpr int KS = KM <= 10 ? 0 : KM <= 14 ? 3 : 4;
         const int64_t nbx = ceil_div(N, KS > 0 ? 128 : 256);
         int64_t s = ceil_div(4096, nbx);
         s = s > Mb ? Mb : (s < 1 ? 1 : s);
