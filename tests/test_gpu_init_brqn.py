@@ -157,7 +157,7 @@ def test_als_matches_cpu_reference_fixture(demo, fixtures, start):
     np.testing.assert_allclose(Q.cpu().numpy(), fixtures["cpu_brqn_als_Q"], atol=1e-6)
 
 
-@pytest.mark.parametrize("K", [3, 7, 10, 11, 13, 16])
+@pytest.mark.parametrize("K", [3, 7, 9, 10, 11, 13, 16])
 def test_grad_hess_matches_oracle(cuda, K):
     rng = np.random.default_rng(K)
     M, N = 523, 300
